@@ -1,0 +1,194 @@
+/*
+ * xbfs.h - C ABI of the MI355X-native XBFS library (libxbfs.so).
+ *
+ * Plain C, plain pointers and sizes.  Every entry point returns an int error
+ * code (XBFS_OK == 0) instead of exiting the process the way the reference's
+ * H_ERR macro does (bfs_single_scan.cuh:11-19).
+ *
+ * Entry points and the reference interface each one replaces:
+ *   xbfs_graph_load / xbfs_graph_free
+ *       graph<int,long,int,int,long,char>::graph(beg, csr, weight)
+ *       graph.hpp:4-104, graph.h:18-36 (same size rules and checks).
+ *   xbfs_graph_write
+ *       the output side of CSRGenerator/tuple_text_to_bin.cpp:359-360, but in
+ *       the int32 adjacency layout the loader reads (bfs_gpu_update.cu:52).
+ *   xbfs_graph_from_edge_text
+ *       CSRGenerator/tuple_text_to_bin.cpp:59-426 (text edge list -> CSR).
+ *   xbfs_gen_rmat / xbfs_gen_er / xbfs_gen_grid
+ *       no reference counterpart (the reference ships no generator); seeded
+ *       inputs for the benchmark configurations.
+ *   xbfs_create / xbfs_destroy
+ *       the device allocations and uploads of bfs_gpu_update.cu:138-240
+ *       (beg_cu, csr_cu, Status_cu, frontier queues, counters).
+ *   xbfs_bfs
+ *       BFS_ControlFlow<int,long>(45 args), bfs_adaptiveFQ_wb_async.cuh:934-1258,
+ *       called at bfs_gpu_update.cu:251: one adaptive direction-optimizing
+ *       BFS from src; per-vertex int32 levels (-1 = unreached) as in the
+ *       Status array copied back at bfs_gpu_update.cu:255; per-level
+ *       ratio/tag/iter records (bfs_adaptiveFQ_wb_async.cuh:1067,1077,1097).
+ *   xbfs_degree_stats
+ *       the degree histogram of bfs_gpu_update.cu:76-89, workload_gap1
+ *       (workload_gap.cuh:27-38) and hub_num (bfs_adaptiveFQ_wb_async.cuh:953-958).
+ *   xbfs_exclusive_scan_i32
+ *       my_prefixsum / IntraBlockSum / InterBlockSum, prefix_sum1.cuh:13-108,
+ *       as a device-only scan (no host round trip).
+ *   xbfs_unvisited_list
+ *       ballot_counting + FQ_gen, bfs_adaptiveFQ_wb_async.cuh:189-284: the
+ *       ascending list of unvisited vertices, built by 64-bit wave ballots.
+ *   xbfs_bfs_batch64, xbfs_bfs_multigraph, xbfs_policy_*, xbfs_mlp_*
+ *       no reference counterpart (batched traversal and the learned
+ *       direction/queue policy that replaces the decision block
+ *       bfs_adaptiveFQ_wb_async.cuh:1047-1078).
+ */
+#ifndef XBFS_H
+#define XBFS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define XBFS_OK 0
+#define XBFS_ERR_ARG 1          /* invalid argument (null, out of range)      */
+#define XBFS_ERR_HIP 2          /* HIP runtime call failed                    */
+#define XBFS_ERR_NOMEM 3        /* host or device allocation failed           */
+#define XBFS_ERR_IO 4           /* file missing, unreadable or truncated      */
+#define XBFS_ERR_FORMAT 5       /* file contents violate the CSR format       */
+#define XBFS_ERR_UNSUPPORTED 6  /* configuration not supported                */
+#define XBFS_ERR_LIMIT 7        /* a size limit was exceeded                  */
+
+const char *xbfs_strerror(int code);
+
+/* ---- host graphs (CSR: beg_pos int64[n+1], adj int32[m]) ---------------- */
+typedef struct {
+    int64_t n;        /* vertices (vert_count)                                */
+    int64_t m;        /* directed CSR entries (edge_count = beg_pos[n])       */
+    int64_t *beg_pos; /* int64[n+1]                                           */
+    int32_t *adj;     /* int32[m]                                             */
+    int8_t *weight;   /* int8[m] when a weight file was read, else NULL       */
+} xbfs_host_graph;
+
+/* Loads <beg_file> (int64 beg_pos, n = filesize/8 - 1) and <csr_file>
+ * (int32 adjacency, beg_pos[n] entries).  weight_file may be NULL or missing
+ * (the reference then prints "Weight file cannot open" and continues). */
+int xbfs_graph_load(const char *beg_file, const char *csr_file,
+                    const char *weight_file, xbfs_host_graph *g);
+/* Writes <prefix>_beg_pos.bin and <prefix>_csr.bin. */
+int xbfs_graph_write(const char *prefix, const xbfs_host_graph *g);
+/* Reads a whitespace-separated "src dst [weight]" text edge list (lines
+ * starting with '#' or '%' skipped).  reverse != 0 also inserts dst->src.
+ * Self-loops dropped, rows sorted and deduplicated. */
+int xbfs_graph_from_edge_text(const char *path, int reverse, xbfs_host_graph *g);
+void xbfs_graph_free(xbfs_host_graph *g);
+
+/* RMAT/Kronecker generator: 2^scale vertices, ef*2^scale sampled edges with
+ * quadrant probabilities a, b, c (d = 1-a-b-c), counter-based RNG seeded by
+ * seed; vertex ids permuted by a seeded shuffle when relabel != 0;
+ * symmetrized, self-loops and duplicates removed, rows sorted ascending. */
+int xbfs_gen_rmat(int32_t scale, int32_t ef, double a, double b, double c,
+                  uint64_t seed, int32_t relabel, xbfs_host_graph *g);
+/* Erdos-Renyi G(n, m): m uniform endpoint pairs, symmetrized, deduplicated. */
+int xbfs_gen_er(int64_t n, int64_t m, uint64_t seed, xbfs_host_graph *g);
+/* rows x cols 4-neighbour grid, vertex id r*cols+c. */
+int xbfs_gen_grid(int32_t rows, int32_t cols, xbfs_host_graph *g);
+
+/* Degree statistics printed by the reference driver. */
+typedef struct {
+    int64_t small;      /* deg <= 32                 (bfs_gpu_update.cu:79)  */
+    int64_t medium;     /* 32 < deg <= 256           (bfs_gpu_update.cu:81)  */
+    int64_t large;      /* deg > 256                 (bfs_gpu_update.cu:84)  */
+    int64_t max_degree; /* workload_gap1             (workload_gap.cuh:27)   */
+    int64_t max_vertex; /* vertex id of max_degree                            */
+    int64_t hub_num;    /* deg > 1000 (theta, bfs_adaptiveFQ_wb_async.cuh:954) */
+    int64_t isolated;   /* deg == 0                                           */
+} xbfs_degree_info;
+int xbfs_degree_stats(const xbfs_host_graph *g, xbfs_degree_info *out);
+
+/* ---- device context ----------------------------------------------------- */
+typedef struct xbfs_ctx xbfs_ctx;
+
+/* Uploads the CSR to device `device` and allocates all traversal state.
+ * The library owns every device buffer until xbfs_destroy. */
+int xbfs_create(int device, const int64_t *beg_pos, const int32_t *adj,
+                int64_t n, int64_t m, xbfs_ctx **out);
+void xbfs_destroy(xbfs_ctx *ctx);
+int64_t xbfs_num_vertices(const xbfs_ctx *ctx);
+int64_t xbfs_num_edges(const xbfs_ctx *ctx);
+
+/* direction / queue modes (the reference's per-level tag in parentheses) */
+#define XBFS_MODE_TD_SF 0 /* top-down scan-free: CAS claim + binned enqueue (tag 0) */
+#define XBFS_MODE_TD_SS 1 /* top-down single-scan: plain stores, no enqueue (tag 1) */
+#define XBFS_MODE_BU 2    /* bottom-up over a frontier bitmap (tag -1)              */
+
+/* policies */
+#define XBFS_POLICY_HEURISTIC 0 /* the reference's alpha / TD_adaptive rule */
+#define XBFS_POLICY_LEARNED 1   /* MLP cost model; falls back to HEURISTIC  */
+#define XBFS_POLICY_FORCE_TD_SF 2
+#define XBFS_POLICY_FORCE_TD_SS 3
+#define XBFS_POLICY_FORCE_BU 4
+#define XBFS_POLICY_SCHEDULE 5  /* mode per level from opts.schedule       */
+
+typedef struct {
+    int32_t block_dim;      /* threads per block, 64..1024, power of two; 0 = 256   */
+    int32_t grid_dim;       /* blocks for frontier kernels; 0 = sized to the GPU    */
+    float alpha;            /* frontier-edge ratio that switches to bottom-up       */
+    float beta;             /* reserved (declared, unused by the reference)         */
+    int32_t padding;        /* accepted for CLI compatibility; work is split per wave */
+    int32_t policy;         /* XBFS_POLICY_*                                         */
+    int32_t persistent;     /* 1 = level loop inside one device-resident kernel      */
+    int32_t want_parent;    /* 1 = also produce the BFS tree (parent_out)            */
+    int32_t bin_thread_max; /* thread bin: 0 < deg <= this; 0 = default              */
+    int32_t bin_wave_max;   /* wave bin: <= this, block bin above; 0 = default       */
+    const int8_t *schedule; /* XBFS_POLICY_SCHEDULE: mode for level L = schedule[min(L, len-1)] */
+    int32_t schedule_len;
+    int32_t time_levels;    /* 1 = per-level device timing in rec[].t_s              */
+} xbfs_opts;
+
+void xbfs_opts_default(xbfs_opts *o);
+
+typedef struct {
+    int32_t level;   /* L                                                     */
+    int32_t mode;    /* XBFS_MODE_* executed                                   */
+    int32_t tag;     /* reference tag: 0 scan-free, 1 single-scan, -1 bottom-up */
+    int32_t policy;  /* policy that decided (LEARNED may fall back)           */
+    int64_t n_f;     /* frontier vertices (level == L)                        */
+    int64_t m_f;     /* frontier edges, the reference's count[L]              */
+    int64_t n_u;     /* unvisited non-isolated vertices at level start        */
+    int64_t m_u;     /* edges of unvisited vertices at level start            */
+    float ratio;     /* (float)m_f / (float)E                                 */
+    float pad;
+    double t_s;      /* device seconds for this level (when time_levels)      */
+} xbfs_level_rec;
+
+typedef struct {
+    double t_bfs_s;   /* device time: level init .. final frontier check      */
+    int32_t n_levels; /* levels executed                                       */
+    int32_t max_level;/* largest level assigned                                */
+    int64_t v_cc;     /* vertices reached                                      */
+    int64_t m_cc;     /* sum of degrees of reached vertices (TEPS numerator)   */
+    double td_s, bu_s, gen_s; /* per-direction device time (when time_levels)  */
+} xbfs_stats;
+
+/* One BFS from src.  level_out: host int32[n] or NULL (levels stay on the
+ * device); parent_out: host int32[n] or NULL (needs opts->want_parent).
+ * rec: host array of rec_cap records or NULL.  st: may be NULL. */
+int xbfs_bfs(xbfs_ctx *ctx, int32_t src, const xbfs_opts *opts,
+             int32_t *level_out, int32_t *parent_out,
+             xbfs_level_rec *rec, int32_t rec_cap, xbfs_stats *st);
+
+/* Device scan used by the frontier generators, exposed for testing:
+ * out[i] = sum(in[0..i-1]) (int64), n up to 2^31-1.  Host pointers. */
+int xbfs_exclusive_scan_i32(xbfs_ctx *ctx, const int32_t *in, int64_t *out, int64_t n);
+
+/* Ascending list of vertices whose level is -1 and degree > 0, for a level
+ * array given on the host (ballot + popcount + device scan).  Returns the
+ * count in *count; list must hold n entries. */
+int xbfs_unvisited_list(xbfs_ctx *ctx, const int32_t *level, int32_t *list,
+                        int64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XBFS_H */

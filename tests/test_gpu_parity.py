@@ -1,0 +1,219 @@
+"""GPU parity: levels from libxbfs (HIP path, through the C ABI) must equal the
+oracle's levels bit for bit; parents must be tree edges one level up; the
+heuristic policy must emit the reference's tag sequence."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import graphs as G
+from xbfs_amd import (Context, Graph, MODE_BU, MODE_TD_SF, MODE_TD_SS, POLICY_FORCE_BU,
+                      POLICY_FORCE_TD_SF, POLICY_FORCE_TD_SS, POLICY_HEURISTIC,
+                      POLICY_SCHEDULE, default_opts)
+
+pytestmark = pytest.mark.gpu
+
+SF, SS, BU = MODE_TD_SF, MODE_TD_SS, MODE_BU
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def seeded_sources(g, k, seed=42):
+    rng = np.random.default_rng(seed)
+    cand = np.flatnonzero(g.degrees() > 0)
+    return rng.choice(cand, size=min(k, cand.size), replace=False).astype(np.int32)
+
+
+def run_check(ctx, g, src, **kw):
+    lv, par, recs, st = ctx.bfs(int(src), want_parent=True, rec_cap=64, **kw)
+    want, _, nl = O.bfs_queue(g.beg_pos, g.adj, int(src))
+    bad = np.flatnonzero(lv != want)
+    assert bad.size == 0, f"src={src} {kw}: {bad.size} mismatches, first {bad[:5]} " \
+                          f"got {lv[bad[:5]]} want {want[bad[:5]]}"
+    assert O.validate(g.beg_pos, g.adj, int(src), lv, par) == 0
+    reached = want >= 0
+    assert st["v_cc"] == reached.sum()
+    assert st["m_cc"] == g.degrees()[reached].sum()
+    assert st["max_level"] == want.max()
+    return lv, recs, st
+
+
+CLOSED = {
+    "path": lambda: G.path(300),
+    "star": lambda: G.star(257),
+    "grid": lambda: G.grid(33, 47),
+    "tree": lambda: G.binary_tree(2047),
+    "complete": lambda: G.complete(70),
+    "disjoint": lambda: G.disjoint(G.path(40), G.star(100), G.csr_from_edges(7, []),
+                                   G.grid(9, 9)),
+}
+POLICIES = [dict(policy=POLICY_HEURISTIC), dict(policy=POLICY_FORCE_TD_SF),
+            dict(policy=POLICY_FORCE_TD_SS), dict(policy=POLICY_FORCE_BU),
+            dict(policy=POLICY_HEURISTIC, alpha=0.01)]
+
+
+@pytest.mark.parametrize("name", sorted(CLOSED))
+def test_closed_forms_all_policies(name):
+    beg, adj = CLOSED[name]()
+    g = Graph(beg, adj)
+    ctx = Context(g)
+    for src in (0, g.n // 2, g.n - 1):
+        for pol in POLICIES:
+            say(name, src, pol)
+            run_check(ctx, g, src, **pol)
+    ctx.close()
+
+
+def test_degree_zero_source():
+    beg, adj = G.disjoint(G.path(10), G.csr_from_edges(3, []))
+    g = Graph(beg, adj)
+    ctx = Context(g)
+    say("isolated source")
+    lv, recs, st = run_check(ctx, g, 11)
+    assert st["n_levels"] == 0 and st["v_cc"] == 1
+
+
+@pytest.fixture(scope="module")
+def rmat10():
+    d = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__),
+                                           "golden", "rmat10_golden.npz"))
+    g = Graph(d["beg_pos"], d["adj"])
+    return g, Context(g), d
+
+
+def test_golden_rmat10(rmat10):
+    g, ctx, d = rmat10
+    for s, want in zip(d["sources"], d["levels"]):
+        for pol in POLICIES:
+            say("rmat10", s, pol)
+            lv, _, _ = run_check(ctx, g, s, **pol)
+            assert (lv == want).all()
+
+
+def test_heuristic_tags_match_reference(rmat10):
+    g, ctx, d = rmat10
+    for key, alpha in (("tags_a0p1", 0.1), ("tags_a0p01", 0.01)):
+        for s, want in zip(d["sources"], d[key]):
+            say("tags", s, alpha)
+            _, recs, st = run_check(ctx, g, s, alpha=alpha)
+            want = want[want != -9]
+            got = np.array([r["tag"] for r in recs], np.int32)
+            k = min(len(got), len(want))
+            assert k == len(want), (got, want)
+            assert (got[:k] == want[:k]).all(), (s, alpha, got, want)
+
+
+SCHEDULES = [
+    [SF, BU, SF], [SF, SF, BU, SF, SF], [BU, BU, SS, SS, SF], [SS, BU, SS],
+    [SF, BU, BU, SF], [SS, SS, SF, BU], [BU, SF, BU, SS], [SF, SS, SF, SS, BU, BU, SF],
+]
+
+
+@pytest.mark.parametrize("sched", SCHEDULES, ids=lambda s: "".join("FSB"[m] for m in s))
+def test_mode_transitions(sched, rmat10):
+    g, ctx, d = rmat10
+    gg = Graph(*G.grid(20, 31))
+    cg = Context(gg)
+    for s in d["sources"][:4]:
+        say("sched", sched, s)
+        run_check(ctx, g, s, policy=POLICY_SCHEDULE, schedule=sched)
+    for s in (0, 317, 619):
+        say("sched grid", sched, s)
+        run_check(cg, gg, s, policy=POLICY_SCHEDULE, schedule=sched)
+    cg.close()
+
+
+def test_random_schedules_rmat12():
+    g = Graph.rmat(12, 16, seed=1012)
+    ctx = Context(g)
+    rng = np.random.default_rng(5)
+    for s in seeded_sources(g, 6):
+        sched = list(rng.integers(0, 3, 12))
+        say("rand sched", s, sched)
+        run_check(ctx, g, s, policy=POLICY_SCHEDULE, schedule=sched)
+
+
+@pytest.mark.parametrize("kw", [dict(bin_thread_max=1, bin_wave_max=4),
+                                dict(bin_thread_max=2, bin_wave_max=2),
+                                dict(block_dim=64, grid_dim=1),
+                                dict(block_dim=1024, grid_dim=3),
+                                dict(block_dim=128, grid_dim=7, bin_thread_max=4,
+                                     bin_wave_max=64)],
+                         ids=["bins1-4", "bins2-2", "b64g1", "b1024g3", "b128g7"])
+def test_launch_shapes_and_bins(kw):
+    g = Graph.rmat(11, 16, seed=31)
+    ctx = Context(g)
+    for s in seeded_sources(g, 3):
+        for pol in POLICIES[:4]:
+            say("shape", kw, s, pol)
+            run_check(ctx, g, s, **kw, **pol)
+
+
+def test_config1_rmat16_16_sources():
+    """RMAT scale 16, ef 16, A/B/C/D = .57/.19/.19/.05, 16 seeded sources."""
+    g = Graph.rmat(16, 16, seed=1016)
+    ctx = Context(g)
+    for s in seeded_sources(g, 16):
+        say("rmat16", s)
+        _, recs, _ = run_check(ctx, g, s)
+        want, _, _ = O.reference_tags(g.beg_pos, g.adj, int(s), 0.1)
+        got = np.array([r["tag"] for r in recs], np.int32)
+        assert (got[: len(want)] == want).all(), (got, want)
+
+
+def test_er_and_grid_small():
+    for g in (Graph.er(20000, 80000, seed=7), Graph.grid(64, 64)):
+        ctx = Context(g)
+        for s in seeded_sources(g, 4):
+            for pol in POLICIES:
+                say("er/grid", g.n, s, pol)
+                run_check(ctx, g, s, **pol)
+        ctx.close()
+
+
+def test_repeat_is_deterministic(rmat10):
+    g, ctx, d = rmat10
+    s = int(d["sources"][0])
+    a = ctx.bfs(s)[0]
+    for _ in range(5):
+        assert (ctx.bfs(s)[0] == a).all()
+
+
+def test_device_scan_matches_exclusive_scan(rmat10):
+    g, ctx, d = rmat10
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 63, 4095, 4096, 4097, 100_003, 3_000_000):
+        say("scan", n)
+        x = rng.integers(0, 1 << 20, n, dtype=np.int32)
+        assert (ctx.exclusive_scan(x) == O.exclusive_scan(x)).all()
+
+
+def test_unvisited_list_is_sorted_unvisited_set():
+    g = Graph.rmat(14, 8, seed=99)
+    ctx = Context(g)
+    rng = np.random.default_rng(1)
+    deg = g.degrees()
+    for frac in (0.0, 0.3, 0.97, 1.0):
+        say("unvisited", frac)
+        lv = np.where(rng.random(g.n) < frac, -1, rng.integers(0, 9, g.n)).astype(np.int32)
+        want = np.flatnonzero((lv == -1) & (deg > 0)).astype(np.int32)
+        assert np.array_equal(ctx.unvisited_list(lv), want)
+
+
+def test_config2_rmat22_full_size():
+    """Full-size RMAT scale 22 (config 2): two sources checked bit-exact
+    against the oracle, every source in a seeded set Graph500-validated."""
+    g = Graph.rmat(22, 16, seed=1022)
+    ctx = Context(g)
+    srcs = seeded_sources(g, 6)
+    for i, s in enumerate(srcs):
+        say("rmat22", s)
+        lv, par, recs, st = ctx.bfs(int(s), want_parent=True, rec_cap=64)
+        assert O.validate(g.beg_pos, g.adj, int(s), lv, par) == 0
+        if i < 2:
+            want, _, _ = O.bfs_queue(g.beg_pos, g.adj, int(s))
+            assert (lv == want).all()

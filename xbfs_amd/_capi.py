@@ -1,0 +1,257 @@
+"""ctypes binding of libxbfs.so (the C ABI declared in include/xbfs.h).
+
+The library is loaded from this package directory only; when it is missing
+every call raises, so nothing here can silently fall back to a CPU path.
+"""
+import ctypes
+import os
+from ctypes import (POINTER, Structure, byref, c_char_p, c_double, c_float, c_int, c_int8,
+                    c_int32, c_int64, c_uint64, c_void_p)
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libxbfs.so")
+
+XBFS_OK = 0
+ERR_NAMES = {
+    1: "XBFS_ERR_ARG", 2: "XBFS_ERR_HIP", 3: "XBFS_ERR_NOMEM", 4: "XBFS_ERR_IO",
+    5: "XBFS_ERR_FORMAT", 6: "XBFS_ERR_UNSUPPORTED", 7: "XBFS_ERR_LIMIT",
+}
+
+MODE_TD_SF, MODE_TD_SS, MODE_BU = 0, 1, 2
+POLICY_HEURISTIC, POLICY_LEARNED = 0, 1
+POLICY_FORCE_TD_SF, POLICY_FORCE_TD_SS, POLICY_FORCE_BU, POLICY_SCHEDULE = 2, 3, 4, 5
+
+
+class XbfsError(RuntimeError):
+    def __init__(self, code, what):
+        self.code = code
+        super().__init__(f"{what}: {ERR_NAMES.get(code, code)}")
+
+
+class HostGraph(Structure):
+    _fields_ = [("n", c_int64), ("m", c_int64), ("beg_pos", POINTER(c_int64)),
+                ("adj", POINTER(c_int32)), ("weight", POINTER(c_int8))]
+
+
+class DegreeInfo(Structure):
+    _fields_ = [("small", c_int64), ("medium", c_int64), ("large", c_int64),
+                ("max_degree", c_int64), ("max_vertex", c_int64), ("hub_num", c_int64),
+                ("isolated", c_int64)]
+
+
+class Opts(Structure):
+    _fields_ = [("block_dim", c_int32), ("grid_dim", c_int32), ("alpha", c_float),
+                ("beta", c_float), ("padding", c_int32), ("policy", c_int32),
+                ("persistent", c_int32), ("want_parent", c_int32),
+                ("bin_thread_max", c_int32), ("bin_wave_max", c_int32),
+                ("schedule", POINTER(c_int8)), ("schedule_len", c_int32),
+                ("time_levels", c_int32)]
+
+
+class LevelRec(Structure):
+    _fields_ = [("level", c_int32), ("mode", c_int32), ("tag", c_int32), ("policy", c_int32),
+                ("n_f", c_int64), ("m_f", c_int64), ("n_u", c_int64), ("m_u", c_int64),
+                ("ratio", c_float), ("pad", c_float), ("t_s", c_double)]
+
+
+class Stats(Structure):
+    _fields_ = [("t_bfs_s", c_double), ("n_levels", c_int32), ("max_level", c_int32),
+                ("v_cc", c_int64), ("m_cc", c_int64), ("td_s", c_double), ("bu_s", c_double),
+                ("gen_s", c_double)]
+
+
+# every symbol include/xbfs.h declares, with its ctypes signature
+SIGNATURES = {
+    "xbfs_strerror": (c_char_p, [c_int]),
+    "xbfs_graph_load": (c_int, [c_char_p, c_char_p, c_char_p, POINTER(HostGraph)]),
+    "xbfs_graph_write": (c_int, [c_char_p, POINTER(HostGraph)]),
+    "xbfs_graph_from_edge_text": (c_int, [c_char_p, c_int, POINTER(HostGraph)]),
+    "xbfs_graph_free": (None, [POINTER(HostGraph)]),
+    "xbfs_gen_rmat": (c_int, [c_int32, c_int32, c_double, c_double, c_double, c_uint64,
+                              c_int32, POINTER(HostGraph)]),
+    "xbfs_gen_er": (c_int, [c_int64, c_int64, c_uint64, POINTER(HostGraph)]),
+    "xbfs_gen_grid": (c_int, [c_int32, c_int32, POINTER(HostGraph)]),
+    "xbfs_degree_stats": (c_int, [POINTER(HostGraph), POINTER(DegreeInfo)]),
+    "xbfs_create": (c_int, [c_int, c_void_p, c_void_p, c_int64, c_int64, POINTER(c_void_p)]),
+    "xbfs_destroy": (None, [c_void_p]),
+    "xbfs_num_vertices": (c_int64, [c_void_p]),
+    "xbfs_num_edges": (c_int64, [c_void_p]),
+    "xbfs_opts_default": (None, [POINTER(Opts)]),
+    "xbfs_bfs": (c_int, [c_void_p, c_int32, POINTER(Opts), c_void_p, c_void_p,
+                         POINTER(LevelRec), c_int32, POINTER(Stats)]),
+    "xbfs_exclusive_scan_i32": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),
+    "xbfs_unvisited_list": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(c_int64)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libxbfs.so from the package directory (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C xbfs_amd/csrc")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != XBFS_OK:
+        raise XbfsError(rc, what)
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Graph:
+    """A host CSR graph: beg_pos int64[n+1], adj int32[m] (numpy arrays)."""
+
+    def __init__(self, beg_pos, adj):
+        self.beg_pos = np.ascontiguousarray(beg_pos, dtype=np.int64)
+        self.adj = np.ascontiguousarray(adj, dtype=np.int32)
+        self.n = int(self.beg_pos.shape[0] - 1)
+        self.m = int(self.adj.shape[0])
+
+    @staticmethod
+    def _from_host(hg):
+        n, m = hg.n, hg.m
+        beg = np.ctypeslib.as_array(hg.beg_pos, shape=(n + 1,)).copy()
+        adj = np.ctypeslib.as_array(hg.adj, shape=(m,)).copy() if m > 0 else np.zeros(0, np.int32)
+        lib().xbfs_graph_free(byref(hg))
+        return Graph(beg, adj)
+
+    @staticmethod
+    def rmat(scale, ef, a=0.57, b=0.19, c=0.19, seed=1, relabel=True):
+        hg = HostGraph()
+        check(lib().xbfs_gen_rmat(scale, ef, a, b, c, seed, int(relabel), byref(hg)), "gen_rmat")
+        return Graph._from_host(hg)
+
+    @staticmethod
+    def er(n, m, seed=7):
+        hg = HostGraph()
+        check(lib().xbfs_gen_er(n, m, seed, byref(hg)), "gen_er")
+        return Graph._from_host(hg)
+
+    @staticmethod
+    def grid(rows, cols):
+        hg = HostGraph()
+        check(lib().xbfs_gen_grid(rows, cols, byref(hg)), "gen_grid")
+        return Graph._from_host(hg)
+
+    @staticmethod
+    def load(beg_file, csr_file, weight_file=None):
+        hg = HostGraph()
+        check(lib().xbfs_graph_load(beg_file.encode(), csr_file.encode(),
+                                    weight_file.encode() if weight_file else None, byref(hg)),
+              "graph_load")
+        return Graph._from_host(hg)
+
+    @staticmethod
+    def from_edge_text(path, reverse=True):
+        hg = HostGraph()
+        check(lib().xbfs_graph_from_edge_text(path.encode(), int(reverse), byref(hg)),
+              "graph_from_edge_text")
+        return Graph._from_host(hg)
+
+    def _host(self):
+        hg = HostGraph()
+        hg.n, hg.m = self.n, self.m
+        hg.beg_pos = self.beg_pos.ctypes.data_as(POINTER(c_int64))
+        hg.adj = self.adj.ctypes.data_as(POINTER(c_int32))
+        hg.weight = POINTER(c_int8)()
+        return hg
+
+    def write(self, prefix):
+        hg = self._host()
+        check(lib().xbfs_graph_write(prefix.encode(), byref(hg)), "graph_write")
+
+    def degree_stats(self):
+        hg = self._host()
+        di = DegreeInfo()
+        check(lib().xbfs_degree_stats(byref(hg), byref(di)), "degree_stats")
+        return {k: getattr(di, k) for k, _ in DegreeInfo._fields_}
+
+    def degrees(self):
+        return np.diff(self.beg_pos)
+
+
+def default_opts(**kw):
+    o = Opts()
+    lib().xbfs_opts_default(byref(o))
+    sched = kw.pop("schedule", None)
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    if sched is not None:
+        arr = (c_int8 * len(sched))(*sched)
+        o.schedule = ctypes.cast(arr, POINTER(c_int8))
+        o.schedule_len = len(sched)
+        o._sched_keepalive = arr
+    return o
+
+
+class Context:
+    """Device context: the CSR uploaded to one GPU plus all traversal state."""
+
+    def __init__(self, graph, device=0):
+        self.graph = graph
+        h = c_void_p()
+        check(lib().xbfs_create(device, _ptr(graph.beg_pos), _ptr(graph.adj), graph.n, graph.m,
+                                byref(h)), "xbfs_create")
+        self.h = h
+        self.n, self.m = graph.n, graph.m
+
+    def close(self):
+        if self.h:
+            lib().xbfs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bfs(self, src, opts=None, want_levels=True, want_parent=False, rec_cap=0, **kw):
+        """One BFS from src.  Returns (levels, parents, records, stats)."""
+        o = opts if opts is not None else default_opts(**kw)
+        lv = np.empty(self.n, np.int32) if want_levels else None
+        par = np.empty(self.n, np.int32) if want_parent else None
+        recs = (LevelRec * rec_cap)() if rec_cap > 0 else None
+        st = Stats()
+        rc = lib().xbfs_bfs(self.h, int(src), byref(o), _ptr(lv) if lv is not None else None,
+                            _ptr(par) if par is not None else None, recs, rec_cap, byref(st))
+        check(rc, "xbfs_bfs")
+        out_recs = []
+        if recs is not None:
+            for i in range(min(rec_cap, st.n_levels)):
+                r = recs[i]
+                out_recs.append({k: getattr(r, k) for k, _ in LevelRec._fields_})
+        stats = {k: getattr(st, k) for k, _ in Stats._fields_}
+        return lv, par, out_recs, stats
+
+    def exclusive_scan(self, x):
+        x = np.ascontiguousarray(x, dtype=np.int32)
+        out = np.empty(x.shape[0], np.int64)
+        check(lib().xbfs_exclusive_scan_i32(self.h, _ptr(x), _ptr(out), x.shape[0]), "scan")
+        return out
+
+    def unvisited_list(self, level):
+        level = np.ascontiguousarray(level, dtype=np.int32)
+        out = np.empty(self.n, np.int32)
+        cnt = c_int64()
+        check(lib().xbfs_unvisited_list(self.h, _ptr(level), _ptr(out), byref(cnt)),
+              "unvisited_list")
+        return out[: cnt.value].copy()

@@ -1,0 +1,522 @@
+// Host-side graph handling for libxbfs: the binary CSR loader/writer, seeded
+// generators (RMAT, Erdos-Renyi, 2-D grid), a text edge-list reader and degree
+// statistics.  Plain C++17 + OpenMP; no device code.
+//
+// File format (the reference's, graph.hpp:4-104 with
+// graph<int,long,int,int,long,char>, bfs_gpu_update.cu:52):
+//   <name>_beg_pos.bin : int64[n+1], n = filesize/8 - 1, beg_pos[n] = m
+//   <name>_csr.bin     : int32[m] neighbour ids
+//   <name>_weight.bin  : int32[m], narrowed to int8 when read (optional)
+#include <algorithm>
+#include <cerrno>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+#include <omp.h>
+#include <sys/stat.h>
+
+#include "xbfs.h"
+#include "xbfs_host_internal.h"
+
+namespace {
+
+int64_t file_size(const char *path)
+{
+    struct stat st;
+    if (stat(path, &st) == 0) return (int64_t)st.st_size;
+    return -1;
+}
+
+template <typename T>
+T *alloc_array(int64_t count)
+{
+    if (count < 0) return nullptr;
+    void *p = nullptr;
+    size_t bytes = sizeof(T) * (size_t)(count > 0 ? count : 1);
+    if (posix_memalign(&p, 4096, bytes) != 0) return nullptr;
+    return (T *)p;
+}
+
+// Reads exactly `count` elements; false on a short read.
+template <typename T>
+bool read_exact(FILE *f, T *dst, int64_t count)
+{
+    const int64_t chunk = (int64_t)1 << 26;
+    int64_t done = 0;
+    while (done < count) {
+        int64_t want = std::min(chunk, count - done);
+        size_t got = fread(dst + done, sizeof(T), (size_t)want, f);
+        if ((int64_t)got != want) return false;
+        done += want;
+    }
+    return true;
+}
+
+template <typename T>
+bool write_all(FILE *f, const T *src, int64_t count)
+{
+    const int64_t chunk = (int64_t)1 << 26;
+    int64_t done = 0;
+    while (done < count) {
+        int64_t want = std::min(chunk, count - done);
+        if ((int64_t)fwrite(src + done, sizeof(T), (size_t)want, f) != want) return false;
+        done += want;
+    }
+    return true;
+}
+
+// ---- counter-based RNG (splitmix64) --------------------------------------
+inline uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+struct SplitMix {
+    uint64_t s;
+    explicit SplitMix(uint64_t seed) : s(seed) {}
+    uint64_t next() { s += 0x9E3779B97F4A7C15ull; return mix64(s); }
+};
+// Independent stream per (seed, index): no sequential dependence between
+// edges, so any thread can regenerate any edge.
+inline uint64_t stream_seed(uint64_t seed, uint64_t index)
+{
+    return mix64(seed ^ mix64(index + 0x632BE59BD9B4E019ull));
+}
+
+struct RmatParams {
+    int scale;
+    uint32_t tA, tAB, tABC; // cumulative thresholds on a 32-bit uniform
+    uint64_t seed;
+};
+
+inline void rmat_edge(const RmatParams &p, uint64_t i, uint32_t &u, uint32_t &v)
+{
+    SplitMix rng(stream_seed(p.seed, i));
+    uint32_t uu = 0, vv = 0;
+    uint64_t r = 0;
+    for (int l = 0; l < p.scale; l++) {
+        uint32_t x;
+        if ((l & 1) == 0) { r = rng.next(); x = (uint32_t)r; }
+        else x = (uint32_t)(r >> 32);
+        int bit = p.scale - 1 - l;
+        if (x < p.tA) {
+        } else if (x < p.tAB) {
+            vv |= 1u << bit;
+        } else if (x < p.tABC) {
+            uu |= 1u << bit;
+        } else {
+            uu |= 1u << bit;
+            vv |= 1u << bit;
+        }
+    }
+    u = uu;
+    v = vv;
+}
+
+uint32_t prob_threshold(double p)
+{
+    if (p <= 0.0) return 0;
+    if (p >= 1.0) return 0xFFFFFFFFu;
+    return (uint32_t)(p * 4294967296.0);
+}
+
+// Builds a symmetric CSR (rows sorted, duplicates and self-loops removed) from
+// an edge generator gen(i, u, v) for i in [0, n_edges).  Edges are
+// partitioned into vertex-range buckets, each bucket sorted independently.
+template <typename Gen>
+int build_symmetric_csr(int64_t n, int64_t n_edges, Gen gen, xbfs_host_graph *g)
+{
+    if (n <= 0 || n > INT32_MAX || n_edges < 0) return XBFS_ERR_ARG;
+    int nthreads = omp_get_max_threads();
+    int64_t nb = 1;
+    while (nb < 4096 && nb * 16384 < n) nb <<= 1;
+    const int64_t span = (n + nb - 1) / nb; // vertices per bucket
+    std::vector<int64_t> cnt((size_t)nthreads * nb, 0);
+    const int64_t chunk = (n_edges + nthreads - 1) / (nthreads > 0 ? nthreads : 1);
+
+    #pragma omp parallel num_threads(nthreads)
+    {
+        int t = omp_get_thread_num();
+        int64_t lo = std::min(n_edges, chunk * t), hi = std::min(n_edges, lo + chunk);
+        int64_t *c = &cnt[(size_t)t * nb];
+        for (int64_t i = lo; i < hi; i++) {
+            uint32_t u, v;
+            gen(i, u, v);
+            if (u == v) continue;
+            c[u / span]++;
+            c[v / span]++;
+        }
+    }
+    // offsets: bucket-major, thread-minor
+    std::vector<int64_t> off((size_t)nthreads * nb, 0), bstart(nb + 1, 0);
+    int64_t acc = 0;
+    for (int64_t b = 0; b < nb; b++) {
+        bstart[b] = acc;
+        for (int t = 0; t < nthreads; t++) {
+            off[(size_t)t * nb + b] = acc;
+            acc += cnt[(size_t)t * nb + b];
+        }
+    }
+    bstart[nb] = acc;
+    const int64_t total = acc;
+    uint64_t *keys = alloc_array<uint64_t>(total);
+    if (!keys) return XBFS_ERR_NOMEM;
+
+    #pragma omp parallel num_threads(nthreads)
+    {
+        int t = omp_get_thread_num();
+        int64_t lo = std::min(n_edges, chunk * t), hi = std::min(n_edges, lo + chunk);
+        int64_t *o = &off[(size_t)t * nb];
+        for (int64_t i = lo; i < hi; i++) {
+            uint32_t u, v;
+            gen(i, u, v);
+            if (u == v) continue;
+            keys[o[u / span]++] = ((uint64_t)u << 32) | v;
+            keys[o[v / span]++] = ((uint64_t)v << 32) | u;
+        }
+    }
+    // per bucket: sort, dedupe, count degrees
+    std::vector<int64_t> uniq(nb, 0);
+    int64_t *beg = alloc_array<int64_t>(n + 1);
+    if (!beg) { free(keys); return XBFS_ERR_NOMEM; }
+    #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int64_t b = 0; b < nb; b++) {
+        uint64_t *k = keys + bstart[b];
+        int64_t len = bstart[b + 1] - bstart[b];
+        std::sort(k, k + len);
+        int64_t w = 0;
+        for (int64_t r = 0; r < len; r++)
+            if (w == 0 || k[r] != k[w - 1]) k[w++] = k[r];
+        uniq[b] = w;
+        int64_t vlo = b * span, vhi = std::min(n, vlo + span);
+        for (int64_t x = vlo; x < vhi; x++) beg[x + 1] = 0;
+        for (int64_t r = 0; r < w; r++) beg[(k[r] >> 32) + 1]++;
+    }
+    std::vector<int64_t> ubase(nb + 1, 0);
+    for (int64_t b = 0; b < nb; b++) ubase[b + 1] = ubase[b] + uniq[b];
+    const int64_t m = ubase[nb];
+    int32_t *adj = alloc_array<int32_t>(m);
+    if (!adj) { free(keys); free(beg); return XBFS_ERR_NOMEM; }
+    beg[0] = 0;
+    #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int64_t b = 0; b < nb; b++) {
+        int64_t vlo = b * span, vhi = std::min(n, vlo + span);
+        int64_t run = ubase[b];
+        for (int64_t x = vlo; x < vhi; x++) {
+            int64_t d = beg[x + 1];
+            beg[x + 1] = run + d;
+            run += d;
+        }
+        const uint64_t *k = keys + bstart[b];
+        int32_t *dst = adj + ubase[b];
+        for (int64_t r = 0; r < uniq[b]; r++) dst[r] = (int32_t)(k[r] & 0xFFFFFFFFu);
+    }
+    free(keys);
+    g->n = n;
+    g->m = m;
+    g->beg_pos = beg;
+    g->adj = adj;
+    g->weight = nullptr;
+    return XBFS_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *xbfs_strerror(int code)
+{
+    switch (code) {
+    case XBFS_OK: return "ok";
+    case XBFS_ERR_ARG: return "invalid argument";
+    case XBFS_ERR_HIP: return "HIP runtime error";
+    case XBFS_ERR_NOMEM: return "out of memory";
+    case XBFS_ERR_IO: return "file missing, unreadable or truncated";
+    case XBFS_ERR_FORMAT: return "malformed CSR";
+    case XBFS_ERR_UNSUPPORTED: return "unsupported configuration";
+    case XBFS_ERR_LIMIT: return "size limit exceeded";
+    default: return "unknown error";
+    }
+}
+
+void xbfs_graph_free(xbfs_host_graph *g)
+{
+    if (!g) return;
+    free(g->beg_pos);
+    free(g->adj);
+    free(g->weight);
+    g->beg_pos = nullptr;
+    g->adj = nullptr;
+    g->weight = nullptr;
+    g->n = g->m = 0;
+}
+
+// graph.hpp:4-104: vert_count = fsize(beg)/sizeof(int64) - 1; the CSR holds
+// beg_pos[vert_count] int32 entries (assert ret==edge_count, graph.hpp:61-62);
+// beg_pos[vert_count] must be > 0 (graph.hpp:37); a missing weight file is not
+// an error (graph.hpp:100).  Beyond the reference we also reject
+// non-monotone offsets and out-of-range neighbour ids, which would make the
+// device kernels read out of bounds.
+int xbfs_graph_load(const char *beg_file, const char *csr_file,
+                    const char *weight_file, xbfs_host_graph *g)
+{
+    if (!beg_file || !csr_file || !g) return XBFS_ERR_ARG;
+    std::memset(g, 0, sizeof(*g));
+    int64_t bsz = file_size(beg_file);
+    if (bsz < 0) return XBFS_ERR_IO;
+    if (bsz < 16 || bsz % 8 != 0) return XBFS_ERR_FORMAT;
+    int64_t n = bsz / 8 - 1;
+    if (n > INT32_MAX) return XBFS_ERR_LIMIT;
+    FILE *f = fopen(beg_file, "rb");
+    if (!f) return XBFS_ERR_IO;
+    int64_t *beg = alloc_array<int64_t>(n + 1);
+    if (!beg) { fclose(f); return XBFS_ERR_NOMEM; }
+    bool ok = read_exact(f, beg, n + 1);
+    fclose(f);
+    if (!ok) { free(beg); return XBFS_ERR_IO; }
+    int64_t m = beg[n];
+    if (m <= 0 || beg[0] != 0) { free(beg); return XBFS_ERR_FORMAT; }
+    bool mono = true;
+    #pragma omp parallel for reduction(&& : mono)
+    for (int64_t i = 0; i < n; i++) mono = mono && (beg[i] <= beg[i + 1]);
+    if (!mono) { free(beg); return XBFS_ERR_FORMAT; }
+    int64_t csz = file_size(csr_file);
+    if (csz < 0) { free(beg); return XBFS_ERR_IO; }
+    if (csz < m * 4) { free(beg); return XBFS_ERR_IO; }
+    f = fopen(csr_file, "rb");
+    if (!f) { free(beg); return XBFS_ERR_IO; }
+    int32_t *adj = alloc_array<int32_t>(m);
+    if (!adj) { fclose(f); free(beg); return XBFS_ERR_NOMEM; }
+    ok = read_exact(f, adj, m);
+    fclose(f);
+    if (!ok) { free(beg); free(adj); return XBFS_ERR_IO; }
+    bool inrange = true;
+    #pragma omp parallel for reduction(&& : inrange)
+    for (int64_t j = 0; j < m; j++) inrange = inrange && (adj[j] >= 0 && adj[j] < n);
+    if (!inrange) { free(beg); free(adj); return XBFS_ERR_FORMAT; }
+    int8_t *w = nullptr;
+    if (weight_file) {
+        FILE *wf = fopen(weight_file, "rb");
+        if (wf) {
+            int32_t *tmp = alloc_array<int32_t>(m);
+            w = alloc_array<int8_t>(m);
+            bool wok = tmp && w && read_exact(wf, tmp, m);
+            fclose(wf);
+            if (!wok) { free(tmp); free(w); free(beg); free(adj); return XBFS_ERR_IO; }
+            for (int64_t j = 0; j < m; j++) w[j] = (int8_t)tmp[j];
+            free(tmp);
+        }
+    }
+    g->n = n;
+    g->m = m;
+    g->beg_pos = beg;
+    g->adj = adj;
+    g->weight = w;
+    return XBFS_OK;
+}
+
+int xbfs_graph_write(const char *prefix, const xbfs_host_graph *g)
+{
+    if (!prefix || !g || !g->beg_pos || (!g->adj && g->m > 0)) return XBFS_ERR_ARG;
+    std::string p(prefix);
+    FILE *f = fopen((p + "_beg_pos.bin").c_str(), "wb");
+    if (!f) return XBFS_ERR_IO;
+    bool ok = write_all(f, g->beg_pos, g->n + 1);
+    if (fclose(f) != 0) ok = false;
+    if (!ok) return XBFS_ERR_IO;
+    f = fopen((p + "_csr.bin").c_str(), "wb");
+    if (!f) return XBFS_ERR_IO;
+    ok = write_all(f, g->adj, g->m);
+    if (fclose(f) != 0) ok = false;
+    return ok ? XBFS_OK : XBFS_ERR_IO;
+}
+
+int xbfs_gen_rmat(int32_t scale, int32_t ef, double a, double b, double c,
+                  uint64_t seed, int32_t relabel, xbfs_host_graph *g)
+{
+    if (!g || scale < 1 || scale > 30 || ef < 1) return XBFS_ERR_ARG;
+    if (a < 0 || b < 0 || c < 0 || a + b + c > 1.0) return XBFS_ERR_ARG;
+    RmatParams p;
+    p.scale = scale;
+    p.tA = prob_threshold(a);
+    p.tAB = prob_threshold(a + b);
+    p.tABC = prob_threshold(a + b + c);
+    p.seed = seed;
+    const int64_t n = (int64_t)1 << scale;
+    const int64_t ne = (int64_t)ef << scale;
+    std::vector<uint32_t> perm;
+    if (relabel) {
+        perm.resize((size_t)n);
+        for (int64_t i = 0; i < n; i++) perm[(size_t)i] = (uint32_t)i;
+        SplitMix rng(mix64(seed ^ 0xA5A5A5A5DEADBEEFull));
+        for (int64_t i = n - 1; i > 0; i--) {
+            uint64_t j = (uint64_t)(((unsigned __int128)rng.next() * (uint64_t)(i + 1)) >> 64);
+            std::swap(perm[(size_t)i], perm[(size_t)j]);
+        }
+    }
+    const uint32_t *pm = relabel ? perm.data() : nullptr;
+    auto gen = [&p, pm](int64_t i, uint32_t &u, uint32_t &v) {
+        rmat_edge(p, (uint64_t)i, u, v);
+        if (pm) { u = pm[u]; v = pm[v]; }
+    };
+    return build_symmetric_csr(n, ne, gen, g);
+}
+
+int xbfs_gen_er(int64_t n, int64_t m, uint64_t seed, xbfs_host_graph *g)
+{
+    if (!g || n < 1 || n > INT32_MAX || m < 0) return XBFS_ERR_ARG;
+    const uint64_t un = (uint64_t)n;
+    auto gen = [seed, un](int64_t i, uint32_t &u, uint32_t &v) {
+        SplitMix rng(stream_seed(seed, (uint64_t)i));
+        u = (uint32_t)(((unsigned __int128)rng.next() * un) >> 64);
+        v = (uint32_t)(((unsigned __int128)rng.next() * un) >> 64);
+    };
+    return build_symmetric_csr(n, m, gen, g);
+}
+
+int xbfs_gen_grid(int32_t rows, int32_t cols, xbfs_host_graph *g)
+{
+    if (!g || rows < 1 || cols < 1 || (int64_t)rows * cols > INT32_MAX) return XBFS_ERR_ARG;
+    const int64_t n = (int64_t)rows * cols;
+    int64_t *beg = alloc_array<int64_t>(n + 1);
+    if (!beg) return XBFS_ERR_NOMEM;
+    beg[0] = 0;
+    // degree of (r,c): number of in-grid 4-neighbours; rows of the prefix are
+    // computed in closed form per row so the fill is parallel.
+    auto deg = [rows, cols](int64_t r, int64_t c) {
+        return (int64_t)(r > 0) + (r < rows - 1) + (c > 0) + (c < cols - 1);
+    };
+    std::vector<int64_t> rowstart((size_t)rows + 1, 0);
+    for (int64_t r = 0; r < rows; r++) {
+        int64_t s = 0;
+        for (int64_t c = 0; c < cols; c++) s += deg(r, c);
+        rowstart[(size_t)r + 1] = rowstart[(size_t)r] + s;
+    }
+    const int64_t m = rowstart[(size_t)rows];
+    int32_t *adj = alloc_array<int32_t>(m);
+    if (!adj) { free(beg); return XBFS_ERR_NOMEM; }
+    #pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < rows; r++) {
+        int64_t pos = rowstart[(size_t)r];
+        for (int64_t c = 0; c < cols; c++) {
+            int64_t id = r * cols + c;
+            if (r > 0) adj[pos++] = (int32_t)(id - cols);
+            if (c > 0) adj[pos++] = (int32_t)(id - 1);
+            if (c < cols - 1) adj[pos++] = (int32_t)(id + 1);
+            if (r < rows - 1) adj[pos++] = (int32_t)(id + cols);
+            beg[id + 1] = pos;
+        }
+    }
+    g->n = n;
+    g->m = m;
+    g->beg_pos = beg;
+    g->adj = adj;
+    g->weight = nullptr;
+    return XBFS_OK;
+}
+
+int xbfs_graph_from_edge_text(const char *path, int reverse, xbfs_host_graph *g)
+{
+    if (!path || !g) return XBFS_ERR_ARG;
+    FILE *f = fopen(path, "r");
+    if (!f) return XBFS_ERR_IO;
+    std::vector<uint32_t> us, vs;
+    char line[4096];
+    int64_t vmax = -1;
+    while (fgets(line, sizeof line, f)) {
+        const char *s = line;
+        while (*s == ' ' || *s == '\t') s++;
+        if (*s == '#' || *s == '%' || *s == '\n' || *s == '\0') continue;
+        char *e1 = nullptr, *e2 = nullptr;
+        errno = 0;
+        long long a = strtoll(s, &e1, 10);
+        if (e1 == s) { fclose(f); return XBFS_ERR_FORMAT; }
+        long long b = strtoll(e1, &e2, 10);
+        if (e2 == e1 || errno != 0 || a < 0 || b < 0 || a > INT32_MAX - 1 || b > INT32_MAX - 1) {
+            fclose(f);
+            return XBFS_ERR_FORMAT;
+        }
+        us.push_back((uint32_t)a);
+        vs.push_back((uint32_t)b);
+        vmax = std::max<int64_t>(vmax, std::max<long long>(a, b));
+    }
+    fclose(f);
+    if (vmax < 0) return XBFS_ERR_FORMAT;
+    const int64_t n = vmax + 1;
+    // directed rows (+ reversed edges when asked), sorted and deduplicated
+    std::vector<uint64_t> keys;
+    keys.reserve(us.size() * (reverse ? 2 : 1));
+    for (size_t i = 0; i < us.size(); i++) {
+        if (us[i] == vs[i]) continue;
+        keys.push_back(((uint64_t)us[i] << 32) | vs[i]);
+        if (reverse) keys.push_back(((uint64_t)vs[i] << 32) | us[i]);
+    }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    int64_t *beg = alloc_array<int64_t>(n + 1);
+    int32_t *adj = alloc_array<int32_t>((int64_t)keys.size());
+    if (!beg || !adj) { free(beg); free(adj); return XBFS_ERR_NOMEM; }
+    for (int64_t i = 0; i <= n; i++) beg[i] = 0;
+    for (size_t i = 0; i < keys.size(); i++) {
+        beg[(keys[i] >> 32) + 1]++;
+        adj[i] = (int32_t)(keys[i] & 0xFFFFFFFFu);
+    }
+    for (int64_t i = 0; i < n; i++) beg[i + 1] += beg[i];
+    g->n = n;
+    g->m = (int64_t)keys.size();
+    g->beg_pos = beg;
+    g->adj = adj;
+    g->weight = nullptr;
+    return XBFS_OK;
+}
+
+int xbfs_degree_stats(const xbfs_host_graph *g, xbfs_degree_info *out)
+{
+    if (!g || !out || !g->beg_pos || g->n < 1) return XBFS_ERR_ARG;
+    int64_t s = 0, md = 0, lg = 0, hub = 0, iso = 0;
+    const int64_t *beg = g->beg_pos;
+    const int64_t n = g->n;
+    #pragma omp parallel for reduction(+ : s, md, lg, hub, iso)
+    for (int64_t i = 0; i < n; i++) {
+        int64_t d = beg[i + 1] - beg[i];
+        if (d <= 32) s++;
+        else if (d <= 256) md++;
+        else lg++;
+        if (d > 1000) hub++;
+        if (d == 0) iso++;
+    }
+    // workload_gap1 keeps the first maximum (strict >, workload_gap.cuh:30)
+    int64_t best = beg[1] - beg[0], bi = 0;
+    for (int64_t i = 1; i < n; i++) {
+        int64_t d = beg[i + 1] - beg[i];
+        if (d > best) { best = d; bi = i; }
+    }
+    out->small = s;
+    out->medium = md;
+    out->large = lg;
+    out->max_degree = best;
+    out->max_vertex = bi;
+    out->hub_num = hub;
+    out->isolated = iso;
+    return XBFS_OK;
+}
+
+int xbfs_check_csr(int64_t n, int64_t m, const int64_t *beg, const int32_t *adj)
+{
+    if (n < 1 || n > INT32_MAX || m < 0 || !beg || (m > 0 && !adj)) return XBFS_ERR_ARG;
+    if (beg[0] != 0 || beg[n] != m) return XBFS_ERR_FORMAT;
+    bool ok = true;
+    #pragma omp parallel for reduction(&& : ok)
+    for (int64_t i = 0; i < n; i++) ok = ok && (beg[i] <= beg[i + 1]);
+    if (!ok) return XBFS_ERR_FORMAT;
+    #pragma omp parallel for reduction(&& : ok)
+    for (int64_t j = 0; j < m; j++) ok = ok && (adj[j] >= 0 && adj[j] < n);
+    return ok ? XBFS_OK : XBFS_ERR_FORMAT;
+}
+
+} // extern "C"
