@@ -1,0 +1,154 @@
+"""XBFS benchmark: adaptive direction-optimizing BFS on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): RMAT scale 24,
+edge factor 16, A/B/C/D = 0.57/0.19/0.19/0.05, seeded, symmetrized,
+self-loops and duplicates removed, vertices relabelled; 64 seeded sources
+drawn from non-isolated vertices.  One step = one BFS from each of the 64
+sources with the graph resident in HBM.
+
+value = harmonic mean over the 64 sources of GTEPS = m_cc / t_bfs / 1e9,
+where m_cc is the sum of degrees of the vertices reached and t_bfs is the
+device time (HIP events) from level initialisation to the final frontier
+check, averaged over the timed steps.
+
+roofline: algorithmic bytes of one traversal, B = 8*m_cc + 20*V_cc + 4*V
+(int32 adjacency word + int32 level probe per traversed edge; int64 offset
+pair + int32 level write per reached vertex; level initialisation), divided
+by the device time of the launch that runs the traversal, against the
+8 TB/s HBM peak.
+
+cpu_baseline: OpenMP top-down BFS (the oracle's CPU baseline) on a bounded
+sample of the same sources, on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+
+
+def hmean(x):
+    x = np.asarray(x, np.float64)
+    return float(len(x) / np.sum(1.0 / x))
+
+
+def pick_sources(deg, k, seed=42):
+    rng = np.random.default_rng(seed)
+    cand = np.flatnonzero(deg > 0)
+    return rng.choice(cand, size=min(k, cand.size), replace=False).astype(np.int32)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--ef", type=int, default=16)
+    ap.add_argument("--sources", type=int, default=64)
+    ap.add_argument("--persistent", type=int, default=-1,
+                    help="1 = device-side level loop, 0 = host-driven, -1 = library default")
+    ap.add_argument("--cpu-sample", type=int, default=4,
+                    help="sources timed for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              f"(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    from xbfs_amd import Context, Graph, default_opts
+    t0 = time.time()
+    g = Graph.rmat(args.scale, args.ef, 0.57, 0.19, 0.19, seed=1000 + args.scale)
+    t_gen = time.time() - t0
+    deg = g.degrees()
+    srcs = pick_sources(deg, args.sources)
+    ctx = Context(g, device=0)
+    kw = {}
+    if args.persistent >= 0:
+        kw["persistent"] = args.persistent
+    opts = default_opts(**kw)
+    print(f"[bench] RMAT-{args.scale} ef{args.ef}: n={g.n} m={g.m} gen {t_gen:.1f}s; "
+          f"{len(srcs)} sources", file=sys.stderr, flush=True)
+
+    def one_step():
+        ts, st_all = [], []
+        for s in srcs:
+            _, _, _, st = ctx.bfs(int(s), opts=opts, want_levels=False)
+            ts.append(st["t_bfs_s"])
+            st_all.append(st)
+        return ts, st_all
+
+    for _ in range(args.warmup):
+        one_step()
+    t_dev = np.zeros(len(srcs))
+    st_ref = None
+    w0 = time.perf_counter()
+    for k in range(args.steps):
+        ts, sts = one_step()
+        t_dev += np.asarray(ts)
+        st_ref = sts
+        print(f"[bench] step {k} done", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - w0
+    t_dev /= max(args.steps, 1)
+    m_cc = np.array([s["m_cc"] for s in st_ref], np.float64)
+    v_cc = np.array([s["v_cc"] for s in st_ref], np.float64)
+    gteps = m_cc / t_dev / 1e9
+    value = hmean(gteps)
+    b_alg = 8 * m_cc + 20 * v_cc + 4 * g.n
+    achieved = float(b_alg.sum() / t_dev.sum())
+
+    # correctness spot check of one source against the oracle validator
+    from oracle import oracle as O
+    s0 = int(srcs[0])
+    lv, par, _, _ = ctx.bfs(s0, opts=opts, want_parent=True)
+    valid = O.validate(g.beg_pos, g.adj, s0, lv, par) == 0
+
+    cpu = None
+    if args.cpu_sample > 0:
+        nthr = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        cg = []
+        for s in srcs[: args.cpu_sample]:
+            lvc, tc = O.omp_td_bfs(g.beg_pos, g.adj, int(s), nthr)
+            cg.append(float(deg[lvc >= 0].sum()) / tc / 1e9)
+        cpu = {"value": hmean(cg), "unit": "GTEPS", "cores": nthr, "kind": "port",
+               "sample": f"OpenMP top-down BFS, first {len(cg)} of the 64 sources, same graph"}
+
+    out = {
+        "metric": "GTEPS (harm. mean, 64 src) RMAT-24 ef16; learned vs heuristic switch; % HBM",
+        "value": round(value, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / max(args.steps, 1) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded RMAT generated in-process)",
+        "config": {"workload": f"RMAT scale {args.scale} ef{args.ef} A/B/C/D=.57/.19/.19/.05, "
+                               f"{len(srcs)} seeded sources, single-source BFS each",
+                   "n": g.n, "m": g.m, "policy": "heuristic (reference alpha rule, alpha=0.1)",
+                   "persistent": int(opts.persistent), "levels_valid": bool(valid)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "kernel": "k_persistent" if opts.persistent else "host-driven level loop",
+                     "launches_per_bfs": 1 if opts.persistent else None},
+        "cpu_baseline": cpu,
+        "gteps_min": round(float(gteps.min()), 3),
+        "gteps_max": round(float(gteps.max()), 3),
+        "graph500_gteps_hmean": round(value / 2, 3),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -58,6 +58,7 @@ extern "C" {
 #define XBFS_ERR_FORMAT 5       /* file contents violate the CSR format       */
 #define XBFS_ERR_UNSUPPORTED 6  /* configuration not supported                */
 #define XBFS_ERR_LIMIT 7        /* a size limit was exceeded                  */
+#define XBFS_ERR_DEVICE 8       /* device-side check failed (barrier timeout) */
 
 const char *xbfs_strerror(int code);
 
@@ -131,19 +132,22 @@ int64_t xbfs_num_edges(const xbfs_ctx *ctx);
 #define XBFS_POLICY_SCHEDULE 5  /* mode per level from opts.schedule       */
 
 typedef struct {
-    int32_t block_dim;      /* threads per block, 64..1024, power of two; 0 = 256   */
-    int32_t grid_dim;       /* blocks for frontier kernels; 0 = sized to the GPU    */
+    int32_t block_dim;      /* accepted for CLI compatibility (64..1024, power of
+                               two); the level phases always run 256-thread blocks */
+    int32_t grid_dim;       /* blocks; 0 = sized to the GPU (persistent: all
+                               co-resident blocks, a smaller value caps it)        */
     float alpha;            /* frontier-edge ratio that switches to bottom-up       */
     float beta;             /* reserved (declared, unused by the reference)         */
     int32_t padding;        /* accepted for CLI compatibility; work is split per wave */
     int32_t policy;         /* XBFS_POLICY_*                                         */
-    int32_t persistent;     /* 1 = level loop inside one device-resident kernel      */
+    int32_t persistent;     /* 1 = level loop inside one cooperative kernel (default) */
     int32_t want_parent;    /* 1 = also produce the BFS tree (parent_out)            */
     int32_t bin_thread_max; /* thread bin: 0 < deg <= this; 0 = default              */
     int32_t bin_wave_max;   /* wave bin: <= this, block bin above; 0 = default       */
     const int8_t *schedule; /* XBFS_POLICY_SCHEDULE: mode for level L = schedule[min(L, len-1)] */
     int32_t schedule_len;
     int32_t time_levels;    /* 1 = per-level device timing in rec[].t_s              */
+    int32_t bin_block_max;  /* block bin: <= this, grid bin above; 0 = default       */
 } xbfs_opts;
 
 void xbfs_opts_default(xbfs_opts *o);

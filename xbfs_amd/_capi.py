@@ -47,7 +47,7 @@ class Opts(Structure):
                 ("persistent", c_int32), ("want_parent", c_int32),
                 ("bin_thread_max", c_int32), ("bin_wave_max", c_int32),
                 ("schedule", POINTER(c_int8)), ("schedule_len", c_int32),
-                ("time_levels", c_int32)]
+                ("time_levels", c_int32), ("bin_block_max", c_int32)]
 
 
 class LevelRec(Structure):

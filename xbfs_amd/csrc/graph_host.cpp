@@ -240,6 +240,7 @@ const char *xbfs_strerror(int code)
     case XBFS_ERR_FORMAT: return "malformed CSR";
     case XBFS_ERR_UNSUPPORTED: return "unsupported configuration";
     case XBFS_ERR_LIMIT: return "size limit exceeded";
+    case XBFS_ERR_DEVICE: return "device-side check failed";
     default: return "unknown error";
     }
 }

@@ -1,6 +1,7 @@
-// C ABI of libxbfs: device context, the host-driven adaptive level loop
-// (BFS_ControlFlow, bfs_adaptiveFQ_wb_async.cuh:934-1258, without the
-// per-level O(V) host work) and the test hooks for the scan / list kernels.
+// C ABI of libxbfs: device context, the two drivers of the adaptive level loop
+// (BFS_ControlFlow, bfs_adaptiveFQ_wb_async.cuh:934-1258, with the decision
+// made from device-side frontier statistics instead of O(V) host scans), and
+// the test hooks for the scan / list kernels.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <climits>
@@ -13,6 +14,7 @@
 #include "xbfs_host_internal.h"
 #include "xbfs_ctx.h"
 #include "xbfs_kernels.hip"
+#include "xbfs_engine.hip"
 
 #define XB_HIP(expr)                                                                   \
     do {                                                                               \
@@ -22,8 +24,6 @@
 #define XB_LAUNCHED() XB_HIP(hipGetLastError())
 
 namespace {
-
-enum { FS_QUEUE = 0, FS_LIST = 1, FS_SWEEP = 2 };
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
@@ -42,8 +42,9 @@ bool pow2_block(int b) { return b >= 64 && b <= 1024 && (b & (b - 1)) == 0; }
 int xb_ctx_free_device(xbfs_ctx *c)
 {
     (void)hipSetDevice(c->device);
-    void *bufs[] = {c->d_beg, c->d_adj, c->d_nz, c->d_level, c->d_parent, c->d_qbuf,
-                    c->d_list, c->d_bits, c->d_tile, c->d_ctl};
+    void *bufs[] = {c->d_beg,   c->d_adj,  c->d_nz,    c->d_vis,  c->d_fbm,   c->d_level,
+                    c->d_parent, c->d_qbuf, c->d_ctl,  c->d_recs, c->d_sched, c->d_mlp,
+                    c->d_list,  c->d_ubits, c->d_tile, c->d_list_cnt};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
@@ -60,19 +61,20 @@ void xbfs_opts_default(xbfs_opts *o)
 {
     if (!o) return;
     std::memset(o, 0, sizeof(*o));
-    o->block_dim = 256;
+    o->block_dim = XB_PBLOCK;
     o->grid_dim = 0;
     o->alpha = 0.1f; // run.sh:36
-    o->beta = 0.1f;  // wb:1018 (unused)
+    o->beta = 0.1f;  // bfs_adaptiveFQ_wb_async.cuh:1018 (declared, unused)
     o->padding = 1;
     o->policy = XBFS_POLICY_HEURISTIC;
-    o->persistent = 0;
+    o->persistent = 1;
     o->want_parent = 0;
     o->bin_thread_max = XB_DEFAULT_T1;
     o->bin_wave_max = XB_DEFAULT_T2;
     o->schedule = nullptr;
     o->schedule_len = 0;
     o->time_levels = 0;
+    o->bin_block_max = XB_DEFAULT_T3;
 }
 
 int64_t xbfs_num_vertices(const xbfs_ctx *c) { return c ? c->n : -1; }
@@ -95,7 +97,9 @@ int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, i
     c->n = n;
     c->m = m;
     c->nwords = ceil_div(n, 64);
-    c->ntiles = ceil_div(c->nwords, XB_TILE_WORDS);
+    c->ntiles_bu = ceil_div(c->nwords, XB_BU_TILE_WORDS);
+    c->fstride = c->ntiles_bu * XB_BU_TILE_WORDS;
+    c->ntiles_list = ceil_div(c->nwords, XB_TILE_WORDS);
     int64_t nz = 0;
     for (int64_t i = 0; i < n; i++) nz += beg[i + 1] > beg[i];
     c->n_nz = nz;
@@ -104,23 +108,40 @@ int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, i
         delete c;
         return code;
     };
+    int clk_khz = 0;
     if (hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device) !=
-        hipSuccess)
+            hipSuccess ||
+        hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess)
         return fail(XBFS_ERR_HIP);
+    if (clk_khz > 0) c->wall_hz = (double)clk_khz * 1e3;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent, XB_PERSIST_BLOCK, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+        return fail(XBFS_ERR_HIP);
+    c->coop_blocks = per_cu * c->num_cu;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(XBFS_ERR_HIP);
     const size_t mm = (size_t)(m > 0 ? m : 1);
+    const size_t nw = (size_t)c->nwords;
     bool ok = hipMalloc(&c->d_beg, sizeof(int64_t) * (size_t)(n + 1)) == hipSuccess &&
               hipMalloc(&c->d_adj, sizeof(int32_t) * mm) == hipSuccess &&
-              hipMalloc(&c->d_nz, sizeof(uint64_t) * (size_t)c->nwords) == hipSuccess &&
+              hipMalloc(&c->d_nz, sizeof(uint64_t) * nw) == hipSuccess &&
+              hipMalloc(&c->d_vis, sizeof(uint64_t) * nw) == hipSuccess &&
+              hipMalloc(&c->d_fbm, sizeof(uint64_t) * (size_t)c->fstride * XB_FRINGS) == hipSuccess &&
               hipMalloc(&c->d_level, sizeof(int32_t) * (size_t)n) == hipSuccess &&
               hipMalloc(&c->d_parent, sizeof(int32_t) * (size_t)n) == hipSuccess &&
-              hipMalloc(&c->d_qbuf, sizeof(int32_t) * (size_t)n * 6) == hipSuccess &&
+              hipMalloc(&c->d_qbuf, sizeof(int32_t) * (size_t)n * XB_QBUFS * XB_BINS) ==
+                  hipSuccess &&
+              hipMalloc(&c->d_ctl, sizeof(XbCtl)) == hipSuccess &&
+              hipMalloc(&c->d_recs, sizeof(XbDevRec) * XB_REC_CAP) == hipSuccess &&
+              hipMalloc(&c->d_sched, XB_REC_CAP) == hipSuccess &&
+              hipMalloc(&c->d_mlp, sizeof(float) * XB_MLP_NPARAM) == hipSuccess &&
               hipMalloc(&c->d_list, sizeof(int32_t) * (size_t)n) == hipSuccess &&
-              hipMalloc(&c->d_bits, sizeof(uint64_t) * (size_t)c->nwords * 3) == hipSuccess &&
-              hipMalloc(&c->d_tile, sizeof(int32_t) * (size_t)c->ntiles * 2) == hipSuccess &&
-              hipMalloc(&c->d_ctl, sizeof(XbDevCtl)) == hipSuccess &&
-              hipHostMalloc(&c->h_ctl, sizeof(XbDevCtl), hipHostMallocDefault) == hipSuccess;
+              hipMalloc(&c->d_ubits, sizeof(uint64_t) * nw) == hipSuccess &&
+              hipMalloc(&c->d_tile, sizeof(int32_t) * (size_t)c->ntiles_list * 2) == hipSuccess &&
+              hipMalloc(&c->d_list_cnt, sizeof(int32_t)) == hipSuccess &&
+              hipHostMalloc(&c->h_ctl, sizeof(XbCtl), hipHostMallocDefault) == hipSuccess;
     if (!ok) return fail(XBFS_ERR_NOMEM);
     if (hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess)
         return fail(XBFS_ERR_HIP);
@@ -151,15 +172,18 @@ int xb_resolve_opts(const xbfs_opts *in, xbfs_opts *o)
 {
     if (in) *o = *in;
     else xbfs_opts_default(o);
-    if (o->block_dim == 0) o->block_dim = 256;
+    if (o->block_dim == 0) o->block_dim = XB_PBLOCK;
     if (!pow2_block(o->block_dim)) return XBFS_ERR_ARG;
     if (o->grid_dim < 0) return XBFS_ERR_ARG;
     if (o->bin_thread_max <= 0) o->bin_thread_max = XB_DEFAULT_T1;
     if (o->bin_wave_max <= 0) o->bin_wave_max = XB_DEFAULT_T2;
-    if (o->bin_wave_max < o->bin_thread_max) return XBFS_ERR_ARG;
+    if (o->bin_block_max <= 0) o->bin_block_max = std::max<int32_t>(XB_DEFAULT_T3, o->bin_wave_max);
+    if (o->bin_wave_max < o->bin_thread_max || o->bin_block_max < o->bin_wave_max)
+        return XBFS_ERR_ARG;
     if (o->policy < XBFS_POLICY_HEURISTIC || o->policy > XBFS_POLICY_SCHEDULE) return XBFS_ERR_ARG;
     if (o->policy == XBFS_POLICY_SCHEDULE) {
-        if (!o->schedule || o->schedule_len <= 0) return XBFS_ERR_ARG;
+        if (!o->schedule || o->schedule_len <= 0 || o->schedule_len > XB_REC_CAP)
+            return XBFS_ERR_ARG;
         for (int i = 0; i < o->schedule_len; i++)
             if (o->schedule[i] < 0 || o->schedule[i] > 2) return XBFS_ERR_ARG;
     }
@@ -177,223 +201,208 @@ static int ensure_events(xbfs_ctx *c, size_t need)
     return XBFS_OK;
 }
 
-// Host-driven level loop: one small D2H copy of the control block per level;
-// everything else (frontier statistics, queues, bitmaps, scans) stays on the
-// device.  Frontier source per level: FS_QUEUE (exact binned queue),
-// FS_LIST (last bottom-up list, filtered on level==L), FS_SWEEP (full sweep).
-static int bfs_host_loop(xbfs_ctx *c, int32_t src, const xbfs_opts &o, xbfs_level_rec *rec,
+static XbArgs make_args(xbfs_ctx *c, int32_t src, const xbfs_opts &o)
+{
+    XbArgs a;
+    a.level = c->d_level;
+    a.parent = o.want_parent ? c->d_parent : nullptr;
+    a.beg = c->d_beg;
+    a.adj = c->d_adj;
+    a.nz = c->d_nz;
+    a.vis = c->d_vis;
+    a.fbm = c->d_fbm;
+    a.qbuf = c->d_qbuf;
+    a.ctl = c->d_ctl;
+    a.recs = c->d_recs;
+    a.rec_cap = XB_REC_CAP;
+    a.sched = c->d_sched;
+    a.sched_len = o.policy == XBFS_POLICY_SCHEDULE ? o.schedule_len : 0;
+    a.mlp = (o.policy == XBFS_POLICY_LEARNED && c->mlp_loaded) ? c->d_mlp : nullptr;
+    a.n = c->n;
+    a.m = c->m;
+    a.nwords = c->nwords;
+    a.ntiles = c->ntiles_bu;
+    a.fstride = c->fstride;
+    a.n_nz = c->n_nz;
+    a.src = src;
+    a.t1 = o.bin_thread_max;
+    a.t2 = o.bin_wave_max;
+    a.t3 = o.bin_block_max;
+    a.policy = o.policy;
+    a.alpha = o.alpha;
+    a.max_levels = (int32_t)std::min<int64_t>(c->n + 1, INT32_MAX - 4);
+    return a;
+}
+
+static void fill_rec(xbfs_level_rec &r, int32_t L, int32_t mode, int32_t htag, int32_t used,
+                     long long n_f, long long m_f, long long n_u, long long m_u, int64_t E,
+                     double t_s)
+{
+    r.level = L;
+    r.mode = mode;
+    r.tag = xbfs_mode_to_tag(mode);
+    r.policy = used;
+    r.n_f = n_f;
+    r.m_f = m_f;
+    r.n_u = n_u;
+    r.m_u = m_u;
+    r.ratio = (float)m_f / (float)E;
+    r.pad = (float)htag;
+    r.t_s = t_s;
+}
+
+// Host-driven level loop: one kernel per phase and one small D2H copy of the
+// control block per level; the decision runs on the host from the same
+// device statistics the persistent kernel reads.
+static int bfs_host_loop(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs_level_rec *rec,
                          int32_t rec_cap, xbfs_stats *st)
 {
-    const bool par = o.want_parent != 0;
-    const int bd = o.block_dim;
-    const int grid = o.grid_dim > 0 ? o.grid_dim : c->num_cu * (2048 / bd);
-    const int32_t t1 = o.bin_thread_max, t2 = o.bin_wave_max;
-    const int64_t n = c->n;
-    XbDevCtl *ctl = c->d_ctl;
-    if (o.time_levels && ensure_events(c, 8) != XBFS_OK) return XBFS_ERR_HIP;
-
+    const int grid = o.grid_dim > 0 ? o.grid_dim : c->num_cu * 8;
+    XbCtl *ctl = c->d_ctl;
+    if (ensure_events(c, 4) != XBFS_OK) return XBFS_ERR_HIP;
     XB_HIP(hipEventRecord(c->ev_start, c->stream));
-    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n / 4 + 1, 256, 8192)), dim3(256), 0, c->stream,
-                       c->d_level, n, -1);
-    if (par)
-        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n / 4 + 1, 256, 8192)), dim3(256), 0,
-                           c->stream, c->d_parent, n, -1);
-    hipLaunchKernelGGL(k_seed, dim3(1), dim3(64), 0, c->stream, c->d_level,
-                       par ? c->d_parent : (int32_t *)nullptr, c->d_beg, ctl, c->q(0, 0),
-                       c->q(0, 1), c->q(0, 2), src, t1, t2);
+    hipLaunchKernelGGL(k_ph_init, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a);
     XB_LAUNCHED();
-
-    int fsrc = FS_QUEUE, cur = 0;
-    XbfsHeurState hs;
-    xbfs_heur_init(&hs);
-    int64_t cum_n = 0, cum_m = 0, pend_n = 0, pend_m = 0;
-    int32_t L = 0, max_level = 0;
+    XbLoop s;
+    xb_loop_init(&s);
     double td_s = 0, bu_s = 0, gen_s = 0;
-    const int sweep_grid = grid_for(n, bd, c->num_cu * 16);
-    for (;; L++) {
-        float gen_ms = 0.0f;
-        if (fsrc == FS_SWEEP) {
+    long long pend_n = 0, pend_m = 0, n_f = 0;
+    for (;;) {
+        const int32_t L = s.L;
+        double sweep_s = 0;
+        if (s.need_sweep) {
             if (o.time_levels) XB_HIP(hipEventRecord(c->ev_pool[0], c->stream));
-            hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, c->stream, ctl, L & 3, -1, cur, -1, 0);
-            XbGenArgs g;
-            g.level = c->d_level;
-            g.beg = c->d_beg;
-            g.n = n;
-            g.L = L;
-            g.t1 = t1;
-            g.t2 = t2;
-            g.q0 = c->q(cur, 0);
-            g.q1 = c->q(cur, 1);
-            g.q2 = c->q(cur, 2);
-            g.cnt = &ctl->qcnt[cur][0];
-            g.slot_n = &ctl->slot_n[L & 3];
-            g.slot_m = &ctl->slot_m[L & 3];
-            hipLaunchKernelGGL(k_sweep_gen, dim3(sweep_grid), dim3(bd), 0, c->stream, g);
+            hipLaunchKernelGGL(k_ph_compact, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L, 1);
             XB_LAUNCHED();
             if (o.time_levels) XB_HIP(hipEventRecord(c->ev_pool[1], c->stream));
-            fsrc = FS_QUEUE;
         }
-        XB_HIP(hipMemcpyAsync(c->h_ctl, ctl, sizeof(XbDevCtl), hipMemcpyDeviceToHost, c->stream));
+        XB_HIP(hipMemcpyAsync(c->h_ctl, ctl, sizeof(XbCtl), hipMemcpyDeviceToHost, c->stream));
         XB_HIP(hipStreamSynchronize(c->stream));
-        if (o.time_levels && gen_ms == 0.0f && c->ev_pool.size() >= 2) {
-            // only meaningful when a sweep ran this level
+        if (o.time_levels && s.need_sweep) {
+            float ms = 0;
+            XB_HIP(hipEventElapsedTime(&ms, c->ev_pool[0], c->ev_pool[1]));
+            sweep_s = ms * 1e-3;
+            gen_s += sweep_s;
         }
-        const XbDevCtl h = *c->h_ctl;
-        const int64_t n_f = (int64_t)h.slot_n[L & 3], m_f = (int64_t)h.slot_m[L & 3];
-        pend_n = (int64_t)h.slot_n[(L + 1) & 3];
-        pend_m = (int64_t)h.slot_m[(L + 1) & 3];
-        if (n_f > 0 && L > max_level) max_level = L;
-        if (pend_n > 0 && L + 1 > max_level) max_level = L + 1;
-        cum_n += n_f;
-        cum_m += m_f;
-        if (m_f == 0 || cum_n + pend_n >= c->n_nz) break;
-        if (L > n) return XBFS_ERR_LIMIT; // a BFS has at most n levels
-
-        const int64_t n_u = c->n_nz - (cum_n + pend_n);
-        const int64_t m_u = c->m - (cum_m + pend_m);
-        int32_t sub = 0;
-        const int32_t htag = xbfs_heur_decide(&hs, L, m_f, c->m, o.alpha, &sub);
-        int32_t mode;
-        switch (o.policy) {
-        case XBFS_POLICY_FORCE_TD_SF: mode = XBFS_MODE_TD_SF; break;
-        case XBFS_POLICY_FORCE_TD_SS: mode = XBFS_MODE_TD_SS; break;
-        case XBFS_POLICY_FORCE_BU: mode = XBFS_MODE_BU; break;
-        case XBFS_POLICY_SCHEDULE:
-            mode = o.schedule[L < o.schedule_len ? L : o.schedule_len - 1];
-            break;
-        default: mode = xbfs_tag_to_mode(htag); break;
-        }
-        if (o.time_levels) {
-            XB_HIP(hipEventRecord(c->ev_pool[2], c->stream));
-        }
-        // slot L+2 and the next queue's counters start at zero this level
-        hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, c->stream, ctl, (L + 2) & 3, -1, cur ^ 1,
-                           (fsrc == FS_LIST && mode != XBFS_MODE_BU) ? cur : -1, 0);
-        if (mode != XBFS_MODE_BU && fsrc == FS_LIST) {
-            XbFilterArgs f;
-            f.list = c->d_list;
-            f.list_cnt = &ctl->list_cnt;
-            f.level = c->d_level;
-            f.beg = c->d_beg;
-            f.L = L;
-            f.t1 = t1;
-            f.t2 = t2;
-            f.carry = mode == XBFS_MODE_TD_SF;
-            f.qc0 = c->q(cur, 0);
-            f.qc1 = c->q(cur, 1);
-            f.qc2 = c->q(cur, 2);
-            f.cnt_c = &ctl->qcnt[cur][0];
-            f.qn0 = c->q(cur ^ 1, 0);
-            f.qn1 = c->q(cur ^ 1, 1);
-            f.qn2 = c->q(cur ^ 1, 2);
-            f.cnt_n = &ctl->qcnt[cur ^ 1][0];
-            hipLaunchKernelGGL(k_list_filter, dim3(grid), dim3(bd), 0, c->stream, f);
-        }
-        if (mode == XBFS_MODE_TD_SF || mode == XBFS_MODE_TD_SS) {
-            XbTdArgs a;
-            a.level = c->d_level;
-            a.parent = c->d_parent;
-            a.beg = c->d_beg;
-            a.adj = c->d_adj;
-            a.qc0 = c->q(cur, 0);
-            a.qc1 = c->q(cur, 1);
-            a.qc2 = c->q(cur, 2);
-            a.cnt_c = &ctl->qcnt[cur][0];
-            a.qn0 = c->q(cur ^ 1, 0);
-            a.qn1 = c->q(cur ^ 1, 1);
-            a.qn2 = c->q(cur ^ 1, 2);
-            a.cnt_n = &ctl->qcnt[cur ^ 1][0];
-            a.slot_n1 = &ctl->slot_n[(L + 1) & 3];
-            a.slot_m1 = &ctl->slot_m[(L + 1) & 3];
-            a.L = L;
-            a.t1 = t1;
-            a.t2 = t2;
-            if (mode == XBFS_MODE_TD_SF) {
-                if (par) hipLaunchKernelGGL((k_td<true, true>), dim3(grid), dim3(bd), 0, c->stream, a);
-                else hipLaunchKernelGGL((k_td<true, false>), dim3(grid), dim3(bd), 0, c->stream, a);
-                cur ^= 1;
-                fsrc = FS_QUEUE;
-            } else {
-                if (par) hipLaunchKernelGGL((k_td<false, true>), dim3(grid), dim3(bd), 0, c->stream, a);
-                else hipLaunchKernelGGL((k_td<false, false>), dim3(grid), dim3(bd), 0, c->stream, a);
-                fsrc = FS_SWEEP;
-            }
+        const XbCtl &h = *c->h_ctl;
+        long long m_f;
+        if (s.need_sweep) {
+            n_f = (long long)h.sw_n[L & 1];
+            m_f = (long long)h.sw_m[L & 1];
         } else {
-            uint64_t *fb = c->d_bits, *nbm = c->d_bits + c->nwords, *ub = c->d_bits + 2 * c->nwords;
-            int32_t *tile_cnt = c->d_tile, *tile_off = c->d_tile + c->ntiles;
-            XbBuCountArgs bc;
-            bc.level = c->d_level;
-            bc.nz = c->d_nz;
-            bc.n = n;
-            bc.nwords = c->nwords;
-            bc.L = L;
-            bc.fb = fb;
-            bc.nb = nbm;
-            bc.ub = ub;
-            bc.tile_cnt = tile_cnt;
-            hipLaunchKernelGGL(k_bu_count, dim3((unsigned)c->ntiles), dim3(256), 0, c->stream, bc);
-            hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, c->stream, tile_cnt, tile_off,
-                               c->ntiles, &ctl->list_cnt);
-            hipLaunchKernelGGL(k_bu_scatter, dim3((unsigned)c->ntiles), dim3(256), 0, c->stream, ub,
-                               c->nwords, tile_off, c->d_list);
-            XbBuArgs b;
-            b.level = c->d_level;
-            b.parent = c->d_parent;
-            b.beg = c->d_beg;
-            b.adj = c->d_adj;
-            b.list = c->d_list;
-            b.list_cnt = &ctl->list_cnt;
-            b.fb = fb;
-            b.nb = nbm;
-            b.slot_n1 = &ctl->slot_n[(L + 1) & 3];
-            b.slot_m1 = &ctl->slot_m[(L + 1) & 3];
-            b.slot_n2 = &ctl->slot_n[(L + 2) & 3];
-            b.slot_m2 = &ctl->slot_m[(L + 2) & 3];
-            b.L = L;
-            if (par) hipLaunchKernelGGL((k_bu<true>), dim3(grid), dim3(bd), 0, c->stream, b);
-            else hipLaunchKernelGGL((k_bu<false>), dim3(grid), dim3(bd), 0, c->stream, b);
-            fsrc = FS_LIST;
+            n_f = (long long)h.slot_n[L & 3];
+            m_f = (long long)h.slot_m[L & 3];
         }
+        pend_n = (long long)h.slot_n[(L + 1) & 3];
+        pend_m = (long long)h.slot_m[(L + 1) & 3];
+        const long long p2_next = (long long)h.slot_p2[(L + 1) & 3];
+        long long n_u = 0, m_u = 0;
+        if (!xb_level_open(s, a, n_f, m_f, pend_n, pend_m, &n_u, &m_u)) break;
+        if (L >= a.max_levels) return XBFS_ERR_LIMIT;
+        float y[XB_MLP_NO];
+        const float *pred = nullptr;
+        if (a.mlp) {
+            float x[XB_MLP_NI];
+            xbfs_mlp_features(L, n_f, m_f, n_u, m_u, s.prev_mode, x);
+            xbfs_mlp_forward1(c->h_mlp.data(), x, y);
+            pred = y;
+        }
+        int32_t htag = 0, used = 0;
+        // host copy of the schedule (a.sched is the device copy)
+        const int32_t mode = xbfs_choose_mode(a.policy, o.schedule, a.sched_len, pred, &s.hs, L,
+                                              m_f, a.m, a.alpha, &htag, &used);
+        if (o.time_levels) XB_HIP(hipEventRecord(c->ev_pool[2], c->stream));
+        hipLaunchKernelGGL(k_ph_prep, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L);
+        if (mode != XBFS_MODE_BU && !s.q_valid && !s.need_sweep)
+            hipLaunchKernelGGL(k_ph_compact, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L, 0);
+        if (mode == XBFS_MODE_TD_SF)
+            hipLaunchKernelGGL(k_ph_td_sf, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L,
+                               (int32_t)(p2_next == 0));
+        else if (mode == XBFS_MODE_TD_SS)
+            hipLaunchKernelGGL(k_ph_td_ss, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L);
+        else
+            hipLaunchKernelGGL(k_ph_bu, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L);
         XB_LAUNCHED();
-        double lvl_s = 0.0;
+        double lvl_s = sweep_s;
         if (o.time_levels) {
             XB_HIP(hipEventRecord(c->ev_pool[3], c->stream));
             XB_HIP(hipEventSynchronize(c->ev_pool[3]));
-            float ms = 0.0f;
+            float ms = 0;
             XB_HIP(hipEventElapsedTime(&ms, c->ev_pool[2], c->ev_pool[3]));
-            lvl_s = ms * 1e-3;
+            lvl_s += ms * 1e-3;
+            if (mode == XBFS_MODE_BU) bu_s += ms * 1e-3;
+            else td_s += ms * 1e-3;
         }
-        if (mode == XBFS_MODE_BU) bu_s += lvl_s;
-        else td_s += lvl_s;
-        (void)gen_ms;
-        if (rec && L < rec_cap) {
-            xbfs_level_rec &r = rec[L];
-            r.level = L;
-            r.mode = mode;
-            r.tag = xbfs_mode_to_tag(mode);
-            r.policy = o.policy;
-            r.n_f = n_f;
-            r.m_f = m_f;
-            r.n_u = n_u;
-            r.m_u = m_u;
-            r.ratio = (float)m_f / (float)c->m;
-            r.pad = (float)htag;
-            r.t_s = lvl_s;
-        }
+        if (rec && L < rec_cap) fill_rec(rec[L], L, mode, htag, used, n_f, m_f, n_u, m_u, c->m, lvl_s);
+        s.q_valid = (mode == XBFS_MODE_TD_SF && p2_next == 0);
+        s.need_sweep = (mode == XBFS_MODE_TD_SS);
+        s.prev_mode = mode;
+        s.L = L + 1;
     }
     XB_HIP(hipEventRecord(c->ev_stop, c->stream));
     XB_HIP(hipEventSynchronize(c->ev_stop));
     float ms = 0.0f;
     XB_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
     if (st) {
+        const int32_t L = s.L;
         st->t_bfs_s = ms * 1e-3;
         st->n_levels = L;
-        st->max_level = max_level;
-        st->v_cc = cum_n + pend_n;
-        st->m_cc = cum_m + pend_m;
+        st->max_level = pend_n > 0 ? L + 1 : (n_f > 0 ? L : L - 1);
+        st->v_cc = s.cum_n + pend_n;
+        st->m_cc = s.cum_m + pend_m;
         st->td_s = td_s;
         st->bu_s = bu_s;
         st->gen_s = gen_s;
+    }
+    return XBFS_OK;
+}
+
+// Persistent driver: the whole level loop in one cooperative launch.
+static int bfs_persistent(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs_level_rec *rec,
+                          int32_t rec_cap, xbfs_stats *st)
+{
+    int grid = c->coop_blocks;
+    if (o.grid_dim > 0 && o.grid_dim < grid) grid = o.grid_dim;
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl);
+    XB_LAUNCHED();
+    XB_HIP(hipEventRecord(c->ev_start, c->stream));
+    XbArgs args = a;
+    void *kargs[] = {&args};
+    XB_HIP(hipLaunchCooperativeKernel((const void *)k_persistent, dim3(grid), dim3(XB_PERSIST_BLOCK),
+                                      kargs, 0, c->stream));
+    XB_HIP(hipEventRecord(c->ev_stop, c->stream));
+    XB_HIP(hipEventSynchronize(c->ev_stop));
+    XB_HIP(hipMemcpy(c->h_ctl, c->d_ctl, sizeof(XbCtl), hipMemcpyDeviceToHost));
+    const XbCtl &h = *c->h_ctl;
+    if (h.error == 1) return XBFS_ERR_DEVICE;
+    if (h.error == 2) return XBFS_ERR_LIMIT;
+    float ms = 0.0f;
+    XB_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+    double td_s = 0, bu_s = 0;
+    const int32_t nrec = std::min<int32_t>(h.n_levels, XB_REC_CAP);
+    if (nrec > 0 && (rec || o.time_levels || st)) {
+        std::vector<XbDevRec> dr((size_t)nrec);
+        XB_HIP(hipMemcpy(dr.data(), c->d_recs, sizeof(XbDevRec) * (size_t)nrec,
+                         hipMemcpyDeviceToHost));
+        for (int32_t L = 0; L < nrec; L++) {
+            const XbDevRec &d = dr[(size_t)L];
+            const double t_s = (double)(d.t1 - d.t0) / c->wall_hz;
+            if (d.mode == XBFS_MODE_BU) bu_s += t_s;
+            else td_s += t_s;
+            if (rec && L < rec_cap)
+                fill_rec(rec[L], L, d.mode, (int8_t)(d.htag & 0xff), (d.htag >> 8) & 0xff, d.n_f, d.m_f,
+                         d.n_u, d.m_u, c->m, t_s);
+        }
+    }
+    if (st) {
+        st->t_bfs_s = ms * 1e-3;
+        st->n_levels = h.n_levels;
+        st->max_level = h.max_level;
+        st->v_cc = h.v_cc;
+        st->m_cc = h.m_cc;
+        st->td_s = td_s;
+        st->bu_s = bu_s;
+        st->gen_s = 0;
     }
     return XBFS_OK;
 }
@@ -410,9 +419,13 @@ int xbfs_bfs(xbfs_ctx *c, int32_t src, const xbfs_opts *opts, int32_t *level_out
     int rc = xb_resolve_opts(opts, &o);
     if (rc) return rc;
     if (parent_out) o.want_parent = 1;
-    if (o.policy == XBFS_POLICY_LEARNED) o.policy = XBFS_POLICY_HEURISTIC;
     XB_HIP(hipSetDevice(c->device));
-    rc = bfs_host_loop(c, src, o, rec, rec_cap, st);
+    if (o.policy == XBFS_POLICY_SCHEDULE)
+        XB_HIP(hipMemcpyAsync(c->d_sched, o.schedule, (size_t)o.schedule_len,
+                              hipMemcpyHostToDevice, c->stream));
+    const XbArgs a = make_args(c, src, o);
+    rc = o.persistent ? bfs_persistent(c, a, o, rec, rec_cap, st)
+                      : bfs_host_loop(c, a, o, rec, rec_cap, st);
     if (rc) return rc;
     if (level_out)
         XB_HIP(hipMemcpy(level_out, c->d_level, sizeof(int32_t) * (size_t)c->n,
@@ -461,8 +474,7 @@ int xbfs_unvisited_list(xbfs_ctx *c, const int32_t *level, int32_t *list, int64_
     if (!c || !level || !list || !count) return XBFS_ERR_ARG;
     XB_HIP(hipSetDevice(c->device));
     XB_HIP(hipMemcpy(c->d_level, level, sizeof(int32_t) * (size_t)c->n, hipMemcpyHostToDevice));
-    uint64_t *ub = c->d_bits + 2 * c->nwords;
-    int32_t *tile_cnt = c->d_tile, *tile_off = c->d_tile + c->ntiles;
+    int32_t *tile_cnt = c->d_tile, *tile_off = c->d_tile + c->ntiles_list;
     XbBuCountArgs bc;
     bc.level = c->d_level;
     bc.nz = c->d_nz;
@@ -471,17 +483,17 @@ int xbfs_unvisited_list(xbfs_ctx *c, const int32_t *level, int32_t *list, int64_
     bc.L = INT_MIN / 2;
     bc.fb = nullptr;
     bc.nb = nullptr;
-    bc.ub = ub;
+    bc.ub = c->d_ubits;
     bc.tile_cnt = tile_cnt;
-    hipLaunchKernelGGL(k_bu_count, dim3((unsigned)c->ntiles), dim3(256), 0, c->stream, bc);
+    hipLaunchKernelGGL(k_bu_count, dim3((unsigned)c->ntiles_list), dim3(256), 0, c->stream, bc);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, c->stream, tile_cnt, tile_off,
-                       c->ntiles, &c->d_ctl->list_cnt);
-    hipLaunchKernelGGL(k_bu_scatter, dim3((unsigned)c->ntiles), dim3(256), 0, c->stream, ub,
-                       c->nwords, tile_off, c->d_list);
+                       c->ntiles_list, c->d_list_cnt);
+    hipLaunchKernelGGL(k_bu_scatter, dim3((unsigned)c->ntiles_list), dim3(256), 0, c->stream,
+                       c->d_ubits, c->nwords, tile_off, c->d_list);
     XB_LAUNCHED();
     XB_HIP(hipStreamSynchronize(c->stream));
     int32_t cnt = 0;
-    XB_HIP(hipMemcpy(&cnt, &c->d_ctl->list_cnt, sizeof(int32_t), hipMemcpyDeviceToHost));
+    XB_HIP(hipMemcpy(&cnt, c->d_list_cnt, sizeof(int32_t), hipMemcpyDeviceToHost));
     if (cnt > 0)
         XB_HIP(hipMemcpy(list, c->d_list, sizeof(int32_t) * (size_t)cnt, hipMemcpyDeviceToHost));
     *count = cnt;

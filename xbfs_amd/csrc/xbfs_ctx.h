@@ -4,33 +4,47 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <vector>
-#include "xbfs_device.h"
+#include "xbfs_engine.h"
 
-#define XB_DEFAULT_T1 32   /* thread bin: 0 < deg <= 32    */
-#define XB_DEFAULT_T2 1024 /* wave bin: deg <= 1024, block bin above */
+#define XB_DEFAULT_T1 32     /* thread bin: 0 < deg <= 32                     */
+#define XB_DEFAULT_T2 1024   /* wave bin: deg <= 1024                          */
+#define XB_DEFAULT_T3 16384  /* block bin: deg <= 16384, grid bin above        */
+#define XB_REC_CAP 4096      /* device-side level records kept per traversal   */
 
 struct xbfs_ctx {
     int device = 0;
     int num_cu = 0;
+    int coop_blocks = 0;        // co-resident blocks of the persistent kernel
+    double wall_hz = 1e8;       // device wall clock rate
     hipStream_t stream = nullptr;
     int64_t n = 0, m = 0;       // vertices, directed edges
     int64_t nwords = 0;         // ceil(n / 64) bitmap words
-    int64_t ntiles = 0;         // ceil(nwords / 64) bottom-up tiles
+    int64_t ntiles_bu = 0;      // bottom-up tiles (XB_BU_TILE_WORDS words each)
+    int64_t fstride = 0;        // words per frontier ring (tile multiple)
     int64_t n_nz = 0;           // vertices with degree > 0
     int64_t *d_beg = nullptr;   // int64[n+1]
     int32_t *d_adj = nullptr;   // int32[m]
     uint64_t *d_nz = nullptr;   // bit v: deg(v) > 0
+    uint64_t *d_vis = nullptr;  // visited bitmap
+    uint64_t *d_fbm = nullptr;  // 4 frontier bitmaps (ring by level & 3)
     int32_t *d_level = nullptr; // int32[n]
     int32_t *d_parent = nullptr;
-    int32_t *d_qbuf = nullptr;  // 2 queue buffers x 3 degree bins x n
-    int32_t *d_list = nullptr;  // bottom-up unvisited list, int32[n]
-    uint64_t *d_bits = nullptr; // frontier, next, unvisited bitmaps (3 x nwords)
-    int32_t *d_tile = nullptr;  // tile counts + offsets (2 x ntiles)
-    XbDevCtl *d_ctl = nullptr;
-    XbDevCtl *h_ctl = nullptr;  // pinned mirror
+    int32_t *d_qbuf = nullptr;  // 3 queue buffers x 4 degree bins x n
+    XbCtl *d_ctl = nullptr;
+    XbCtl *h_ctl = nullptr;     // pinned mirror
+    XbDevRec *d_recs = nullptr; // XB_REC_CAP records
+    int8_t *d_sched = nullptr;  // schedule copy (XB_REC_CAP entries)
+    float *d_mlp = nullptr;     // learned policy weights
+    int mlp_loaded = 0;
+    std::vector<float> h_mlp;   // host copy (the host-driven loop decides on the CPU)
+    // ordered unvisited list / generic scan scratch
+    int32_t *d_list = nullptr;
+    uint64_t *d_ubits = nullptr;
+    int32_t *d_tile = nullptr;
+    int32_t *d_list_cnt = nullptr;
+    int64_t ntiles_list = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::vector<hipEvent_t> ev_pool;
-    int32_t *q(int b, int k) { return d_qbuf + ((int64_t)b * 3 + k) * n; }
 };
 
 int xb_ctx_free_device(xbfs_ctx *c);

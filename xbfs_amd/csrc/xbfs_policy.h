@@ -109,4 +109,104 @@ XBFS_HD inline int32_t xbfs_mode_to_tag(int32_t mode)
     return mode == 0 ? 0 : (mode == 1 ? 1 : -1);
 }
 
+// ---- learned cost model ---------------------------------------------------
+// FP32 MLP 8 -> 32 -> 32 -> 3 (ReLU), parameters packed as
+// W1[32][8], b1[32], W2[32][32], b2[32], W3[3][32], b3[3] (1443 floats).
+// Output j is the predicted log2 time (us) of mode j (XBFS_MODE_* order:
+// 0 top-down scan-free, 1 top-down single-scan, 2 bottom-up).
+#define XB_MLP_NI 8
+#define XB_MLP_NH 32
+#define XB_MLP_NO 3
+#define XB_MLP_NPARAM (XB_MLP_NH * XB_MLP_NI + XB_MLP_NH + XB_MLP_NH * XB_MLP_NH + XB_MLP_NH + \
+                       XB_MLP_NO * XB_MLP_NH + XB_MLP_NO)
+
+XBFS_HD inline float xb_log2p1(long long v)
+{
+    // log2(1+v) without libm: exponent from the leading bit, mantissa by a
+    // cubic fit of log2(1+f) on [0,1) (max error ~1e-3, plenty for features)
+    double x = (double)v + 1.0;
+    int e = 0;
+    while (x >= 2.0) { x *= 0.5; e++; }
+    double f = x - 1.0;
+    double l = f * (1.4425 - f * (0.7021 - f * 0.2596));
+    return (float)(e + l);
+}
+
+// Features of level L: log2 sizes of the frontier and the unvisited part,
+// the level index, the previous direction and whether an exact queue is
+// available (previous level was top-down scan-free), and a constant.
+XBFS_HD inline void xbfs_mlp_features(int32_t L, long long n_f, long long m_f, long long n_u,
+                                      long long m_u, int32_t prev_mode, float *x)
+{
+    x[0] = xb_log2p1(n_f) * (1.0f / 32.0f);
+    x[1] = xb_log2p1(m_f) * (1.0f / 32.0f);
+    x[2] = xb_log2p1(m_u) * (1.0f / 32.0f);
+    x[3] = xb_log2p1(n_u) * (1.0f / 32.0f);
+    x[4] = (float)(L < 64 ? L : 64) * (1.0f / 16.0f);
+    x[5] = prev_mode == 2 ? 1.0f : (prev_mode < 0 ? -1.0f : 0.0f);
+    x[6] = prev_mode == 0 ? 1.0f : 0.0f;
+    x[7] = 1.0f;
+}
+
+XBFS_HD inline void xbfs_mlp_forward1(const float *p, const float *x, float *y)
+{
+    const float *W1 = p, *b1 = W1 + XB_MLP_NH * XB_MLP_NI;
+    const float *W2 = b1 + XB_MLP_NH, *b2 = W2 + XB_MLP_NH * XB_MLP_NH;
+    const float *W3 = b2 + XB_MLP_NH, *b3 = W3 + XB_MLP_NO * XB_MLP_NH;
+    float h1[XB_MLP_NH], h2[XB_MLP_NH];
+    for (int o = 0; o < XB_MLP_NH; o++) {
+        float s = b1[o];
+        for (int i = 0; i < XB_MLP_NI; i++) s += W1[o * XB_MLP_NI + i] * x[i];
+        h1[o] = s > 0.0f ? s : 0.0f;
+    }
+    for (int o = 0; o < XB_MLP_NH; o++) {
+        float s = b2[o];
+        for (int i = 0; i < XB_MLP_NH; i++) s += W2[o * XB_MLP_NH + i] * h1[i];
+        h2[o] = s > 0.0f ? s : 0.0f;
+    }
+    for (int o = 0; o < XB_MLP_NO; o++) {
+        float s = b3[o];
+        for (int i = 0; i < XB_MLP_NH; i++) s += W3[o * XB_MLP_NH + i] * h2[i];
+        y[o] = s;
+    }
+}
+
+// Mode for level L under `policy` (XBFS_POLICY_* values 0..5).  The
+// reference heuristic state always advances, so *htag reports the reference's
+// tag for this level whatever the policy; *used is the policy that decided.
+// pred: the learned model's three predicted costs for this level (NULL when
+// no model is loaded); LEARNED falls back to HEURISTIC without a model or on
+// a non-finite prediction.
+XBFS_HD inline int32_t xbfs_choose_mode(int32_t policy, const int8_t *sched, int32_t sched_len,
+                                        const float *pred, XbfsHeurState *hs, int32_t L,
+                                        long long m_f, long long E, float alpha,
+                                        int32_t *htag, int32_t *used)
+{
+    int32_t sub = 0;
+    const int32_t t = xbfs_heur_decide(hs, L, m_f, E, alpha, &sub);
+    *htag = t;
+    *used = policy;
+    switch (policy) {
+    case 2: return 0;
+    case 3: return 1;
+    case 4: return 2;
+    case 5: return sched[L < sched_len ? L : sched_len - 1];
+    case 1:
+        if (pred) {
+            bool finite = true;
+            for (int j = 0; j < XB_MLP_NO; j++)
+                if (!(pred[j] > -1e30f && pred[j] < 1e30f)) finite = false;
+            if (finite) {
+                int32_t best = 0;
+                for (int j = 1; j < XB_MLP_NO; j++)
+                    if (pred[j] < pred[best]) best = j;
+                return best;
+            }
+        }
+        *used = 0;
+        return xbfs_tag_to_mode(t);
+    default: *used = 0; return xbfs_tag_to_mode(t);
+    }
+}
+
 #endif
