@@ -192,6 +192,47 @@ int xbfs_exclusive_scan_i32(xbfs_ctx *ctx, const int32_t *in, int64_t *out, int6
 int xbfs_unvisited_list(xbfs_ctx *ctx, const int32_t *level, int32_t *list,
                         int64_t *count);
 
+/* ---- learned direction / queue policy ------------------------------------
+ * No reference counterpart: a cost model that replaces the decision block
+ * bfs_adaptiveFQ_wb_async.cuh:1047-1078 when loaded (policy LEARNED).
+ * FP32 MLP 8 -> 32 -> 32 -> 3 (ReLU), XBFS_MLP_NPARAM floats packed as
+ * W1[32][8], b1[32], W2[32][32], b2[32], W3[3][32], b3[3]; output j is the
+ * predicted log2 microseconds of mode j at the level (XBFS_MODE_* order). */
+#define XBFS_MLP_NPARAM 1443
+
+/* The 8 features of a level (the same function the kernels evaluate). */
+int xbfs_policy_features(int32_t level, int64_t n_f, int64_t m_f, int64_t n_u, int64_t m_u,
+                         int32_t prev_mode, float *x8);
+/* Loads weights into a context (copied; used by policy LEARNED). */
+int xbfs_policy_load(xbfs_ctx *ctx, const float *weights, int32_t nweights);
+
+/* HIP kernels of the model, host pointers in and out, on GPU `device`. */
+int xbfs_mlp_forward(int device, const float *params, const float *x, int32_t n, float *out);
+int xbfs_mlp_loss_grad(int device, const float *params, const float *x, const float *y,
+                       const float *mask, int32_t n, float *grad, float *loss);
+/* Full-batch Adam for `steps` steps; params is the initial value on entry
+ * and the trained value on return; loss_curve (steps floats) may be NULL. */
+int xbfs_mlp_train(int device, float *params, const float *x, const float *y, const float *mask,
+                   int32_t n, int32_t steps, float lr, float *loss_curve);
+
+/* Per-level probe: at every level the three modes are each run `reps` times
+ * from a device snapshot of the traversal state and timed; then the level is
+ * executed with the mode `opts->policy` picks (follow_best = 0) or with the
+ * fastest measured mode (follow_best = 1, the per-level oracle-best
+ * schedule).  Host-driven loop; parents are not produced. */
+typedef struct {
+    int32_t level;
+    int32_t prev_mode;   /* -1 at level 0                                     */
+    int32_t mode;        /* mode executed                                     */
+    int32_t htag;        /* reference heuristic tag at this level              */
+    int64_t n_f, m_f, n_u, m_u;
+    double t_mode[3];    /* seconds per mode (median of reps), XBFS_MODE_* order.
+                            TD single-scan includes the frontier sweep the next
+                            level then needs. */
+} xbfs_probe_rec;
+int xbfs_bfs_probe(xbfs_ctx *ctx, int32_t src, const xbfs_opts *opts, int32_t follow_best,
+                   int32_t reps, xbfs_probe_rec *rec, int32_t rec_cap, xbfs_stats *st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,122 @@
+"""Learned direction/queue policy: features (CPU), HIP MLP kernels against
+the double-precision oracle, probe records and the LEARNED policy path (GPU)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from xbfs_amd import _capi as C
+from xbfs_amd import policy as P
+from xbfs_amd import Context, Graph, POLICY_HEURISTIC, POLICY_LEARNED
+
+
+def test_features_closed_form():
+    x = C.policy_features(0, 0, 0, 0, 0, -1)
+    assert np.allclose(x, [0, 0, 0, 0, 0, -1, 0, 1])
+    x = C.policy_features(3, 1023, 4095, (1 << 20) - 1, 15, 2)
+    # log2(1+v)/32 at exact powers of two, level/16, previous mode BU
+    assert np.allclose(x, [10 / 32, 12 / 32, 4 / 32, 20 / 32, 3 / 16, 1, 0, 1], atol=2e-5)
+    x = C.policy_features(1, 5, 5, 5, 5, 0)
+    assert x[5] == 0 and x[6] == 1
+    # log2(1+v) approximation stays within 1e-4 of the true value
+    for v in (2, 3, 6, 100, 12345, 10 ** 9, 2 ** 40 + 12345):
+        assert abs(C.policy_features(0, v, 0, 0, 0, 1)[0] * 32 - np.log2(1 + v)) < 1e-4
+
+
+def test_train_specs_are_seeded_and_in_range():
+    s = P.train_specs(200)
+    assert s == P.train_specs(200)
+    assert all(18 <= t["scale"] <= 22 and 8 <= t["ef"] <= 32 and 0.45 <= t["a"] <= 0.62
+               for t in s)
+    assert all(abs(t["a"] + t["b"] + t["c"] + 0.05 - 1) < 1e-12 for t in s)
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, 8)).astype(np.float32)
+    y = rng.standard_normal((n, 3)).astype(np.float32)
+    m = (rng.random((n, 3)) < 0.8).astype(np.float32)
+    return x, y, m
+
+
+@pytest.mark.gpu
+def test_mlp_forward_and_grad_match_oracle():
+    rng = np.random.default_rng(7)
+    p = (rng.standard_normal(C.MLP_NPARAM) * 0.3).astype(np.float32)
+    for n in (1, 63, 64, 1000, 5000):
+        x, y, m = _data(n, n)
+        out = C.mlp_forward(p, x)
+        want = O.mlp_forward(p, x)
+        assert np.allclose(out, want, rtol=1e-4, atol=1e-4), n
+        loss, g = C.mlp_loss_grad(p, x, y, m)
+        wl, wg = O.mlp_loss_grad(p, x, y, m)
+        assert abs(loss - wl) <= 1e-4 * max(1, abs(wl)), (n, loss, wl)
+        # FP32 accumulation vs double: 1e-4 relative to the gradient's scale
+        assert np.max(np.abs(g - wg)) <= 1e-4 * max(1.0, np.max(np.abs(wg))), n
+
+
+@pytest.mark.gpu
+def test_mlp_training_reduces_loss():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((2000, 8)).astype(np.float32)
+    # a smooth target the network can fit
+    y = np.stack([x[:, 0] + 0.5 * x[:, 1], np.abs(x[:, 2]), x[:, 3] * x[:, 4] * 0.3], 1)
+    y = y.astype(np.float32)
+    m = np.ones_like(y)
+    p0 = P.init_params(0)
+    l0, _ = C.mlp_loss_grad(p0, x, y, m)
+    p, curve = C.mlp_train(p0, x, y, m, 1500, 3e-3)
+    l1, _ = C.mlp_loss_grad(p, x, y, m)
+    assert abs(curve[0] - l0) < 1e-4 * max(1, l0)
+    assert l1 < 0.1 * l0 and curve[-1] < curve[0]
+
+
+@pytest.mark.gpu
+def test_probe_records_and_learned_path_exact():
+    g = Graph.rmat(14, 16, seed=1014)
+    ctx = Context(g)
+    srcs = P.seeded_sources(g.degrees(), 3)
+    for s in srcs:
+        s = int(s)
+        want, _, _ = O.bfs_queue(g.beg_pos, g.adj, s)
+        recs, st = ctx.probe(s, reps=1)
+        assert st["v_cc"] == (want >= 0).sum()
+        cnt, nv, _ = O.level_counts(g.beg_pos, want)
+        for r in recs:
+            assert r["m_f"] == cnt[r["level"]] and r["n_f"] == nv[r["level"]]
+            assert all(t > 0 for t in r["t_mode"])
+        best, st2 = ctx.probe(s, follow_best=True, reps=1)
+        assert all(r["mode"] == int(np.argmin(r["t_mode"])) for r in best)
+        # learned policy without weights falls back to the heuristic
+        _, _, rl, _ = ctx.bfs(s, rec_cap=64, policy=POLICY_LEARNED)
+        _, _, rh, _ = ctx.bfs(s, rec_cap=64, policy=POLICY_HEURISTIC)
+        assert [r["mode"] for r in rl] == [r["mode"] for r in rh]
+    # with (random) weights the learned path still yields exact levels
+    rng = np.random.default_rng(1)
+    ctx.load_policy((rng.standard_normal(C.MLP_NPARAM) * 0.5).astype(np.float32))
+    for s in srcs:
+        for pers in (1, 0):
+            lv, par, _, _ = ctx.bfs(int(s), want_parent=True, policy=POLICY_LEARNED,
+                                    persistent=pers)
+            want, _, _ = O.bfs_queue(g.beg_pos, g.adj, int(s))
+            assert (lv == want).all()
+            assert O.validate(g.beg_pos, g.adj, int(s), lv, par) == 0
+
+
+@pytest.mark.gpu
+def test_trained_policy_gain_on_held_out_graph():
+    """The committed weights (trained by xbfs_amd/train_policy.py on RMAT
+    scale 18-22) against the heuristic on a held-out graph: reports the
+    end-to-end speedup and requires the learned policy not to be slower."""
+    if not os.path.exists(P.WEIGHTS):
+        pytest.skip("no trained weights committed yet")
+    g = Graph.rmat(20, 16, seed=777)
+    ctx = Context(g)
+    srcs = P.seeded_sources(g.degrees(), 8, seed=9)
+    tot = P.evaluate(ctx, srcs, P.load(), reps=5)
+    gain = tot["heuristic"] / tot["learned"]
+    print(f"[policy] held-out RMAT-20: heuristic {tot['heuristic']*1e3:.3f} ms, learned "
+          f"{tot['learned']*1e3:.3f} ms, oracle-best {tot['oracle_best']*1e3:.3f} ms, "
+          f"gain {gain:.3f}x", flush=True)
+    assert gain >= 0.97

@@ -62,6 +62,14 @@ class Stats(Structure):
                 ("gen_s", c_double)]
 
 
+class ProbeRec(Structure):
+    _fields_ = [("level", c_int32), ("prev_mode", c_int32), ("mode", c_int32), ("htag", c_int32),
+                ("n_f", c_int64), ("m_f", c_int64), ("n_u", c_int64), ("m_u", c_int64),
+                ("t_mode", c_double * 3)]
+
+
+MLP_NPARAM = 1443
+
 # every symbol include/xbfs.h declares, with its ctypes signature
 SIGNATURES = {
     "xbfs_strerror": (c_char_p, [c_int]),
@@ -83,6 +91,16 @@ SIGNATURES = {
                          POINTER(LevelRec), c_int32, POINTER(Stats)]),
     "xbfs_exclusive_scan_i32": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),
     "xbfs_unvisited_list": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(c_int64)]),
+    "xbfs_policy_features": (c_int, [c_int32, c_int64, c_int64, c_int64, c_int64, c_int32,
+                                     c_void_p]),
+    "xbfs_policy_load": (c_int, [c_void_p, c_void_p, c_int32]),
+    "xbfs_mlp_forward": (c_int, [c_int, c_void_p, c_void_p, c_int32, c_void_p]),
+    "xbfs_mlp_loss_grad": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
+                                   c_void_p, c_void_p]),
+    "xbfs_mlp_train": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32,
+                               c_float, c_void_p]),
+    "xbfs_bfs_probe": (c_int, [c_void_p, c_int32, POINTER(Opts), c_int32, c_int32,
+                               POINTER(ProbeRec), c_int32, POINTER(Stats)]),
 }
 
 _lib = None
@@ -242,6 +260,25 @@ class Context:
         stats = {k: getattr(st, k) for k, _ in Stats._fields_}
         return lv, par, out_recs, stats
 
+    def load_policy(self, weights):
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        check(lib().xbfs_policy_load(self.h, _ptr(w), w.shape[0]), "policy_load")
+
+    def probe(self, src, follow_best=False, reps=3, rec_cap=256, **kw):
+        """Per-level timing of all three modes (see xbfs_bfs_probe)."""
+        o = default_opts(**kw)
+        recs = (ProbeRec * rec_cap)()
+        st = Stats()
+        check(lib().xbfs_bfs_probe(self.h, int(src), byref(o), int(follow_best), int(reps), recs,
+                                   rec_cap, byref(st)), "xbfs_bfs_probe")
+        out = []
+        for i in range(min(rec_cap, st.n_levels)):
+            r = recs[i]
+            d = {k: getattr(r, k) for k, _ in ProbeRec._fields_ if k != "t_mode"}
+            d["t_mode"] = [r.t_mode[j] for j in range(3)]
+            out.append(d)
+        return out, {k: getattr(st, k) for k, _ in Stats._fields_}
+
     def exclusive_scan(self, x):
         x = np.ascontiguousarray(x, dtype=np.int32)
         out = np.empty(x.shape[0], np.int64)
@@ -255,3 +292,38 @@ class Context:
         check(lib().xbfs_unvisited_list(self.h, _ptr(level), _ptr(out), byref(cnt)),
               "unvisited_list")
         return out[: cnt.value].copy()
+
+
+def policy_features(level, n_f, m_f, n_u, m_u, prev_mode):
+    x = np.empty(8, np.float32)
+    check(lib().xbfs_policy_features(int(level), int(n_f), int(m_f), int(n_u), int(m_u),
+                                     int(prev_mode), _ptr(x)), "policy_features")
+    return x
+
+
+def mlp_forward(params, x, device=0):
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty((x.shape[0], 3), np.float32)
+    check(lib().xbfs_mlp_forward(device, _ptr(params), _ptr(x), x.shape[0], _ptr(out)),
+          "mlp_forward")
+    return out
+
+
+def mlp_loss_grad(params, x, y, mask, device=0):
+    params = np.ascontiguousarray(params, np.float32)
+    x, y, mask = (np.ascontiguousarray(a, np.float32) for a in (x, y, mask))
+    g = np.empty(MLP_NPARAM, np.float32)
+    loss = np.empty(1, np.float32)
+    check(lib().xbfs_mlp_loss_grad(device, _ptr(params), _ptr(x), _ptr(y), _ptr(mask),
+                                   x.shape[0], _ptr(g), _ptr(loss)), "mlp_loss_grad")
+    return float(loss[0]), g
+
+
+def mlp_train(params, x, y, mask, steps, lr, device=0):
+    p = np.array(params, dtype=np.float32, copy=True)
+    x, y, mask = (np.ascontiguousarray(a, np.float32) for a in (x, y, mask))
+    curve = np.empty(steps, np.float32)
+    check(lib().xbfs_mlp_train(device, _ptr(p), _ptr(x), _ptr(y), _ptr(mask), x.shape[0],
+                               int(steps), float(lr), _ptr(curve)), "mlp_train")
+    return p, curve

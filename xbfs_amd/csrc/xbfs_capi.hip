@@ -15,6 +15,7 @@
 #include "xbfs_ctx.h"
 #include "xbfs_kernels.hip"
 #include "xbfs_engine.hip"
+#include "xbfs_mlp.hip"
 
 #define XB_HIP(expr)                                                                   \
     do {                                                                               \
@@ -44,7 +45,8 @@ int xb_ctx_free_device(xbfs_ctx *c)
     (void)hipSetDevice(c->device);
     void *bufs[] = {c->d_beg,   c->d_adj,  c->d_nz,    c->d_vis,  c->d_fbm,   c->d_level,
                     c->d_parent, c->d_qbuf, c->d_ctl,  c->d_recs, c->d_sched, c->d_mlp,
-                    c->d_list,  c->d_ubits, c->d_tile, c->d_list_cnt};
+                    c->d_list,  c->d_ubits, c->d_tile, c->d_list_cnt, c->s_level,
+                    c->s_vis,   c->s_fbm,   c->s_qbuf, c->s_ctl};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
@@ -251,6 +253,24 @@ static void fill_rec(xbfs_level_rec &r, int32_t L, int32_t mode, int32_t htag, i
     r.t_s = t_s;
 }
 
+// Enqueues the kernels of level L in `mode` on the context stream: zeroing
+// schedule, the frontier queue rebuild a top-down mode needs, the expansion.
+static hipError_t launch_level(xbfs_ctx *c, const XbArgs &a, int grid, int32_t L, int32_t mode,
+                               int32_t q_valid, int32_t need_sweep, bool enqueue)
+{
+    hipLaunchKernelGGL(k_ph_prep, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L);
+    if (mode != XBFS_MODE_BU && !q_valid && !need_sweep)
+        hipLaunchKernelGGL(k_ph_compact, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L, 0);
+    if (mode == XBFS_MODE_TD_SF)
+        hipLaunchKernelGGL(k_ph_td_sf, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L,
+                           (int32_t)enqueue);
+    else if (mode == XBFS_MODE_TD_SS)
+        hipLaunchKernelGGL(k_ph_td_ss, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L);
+    else
+        hipLaunchKernelGGL(k_ph_bu, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L);
+    return hipGetLastError();
+}
+
 // Host-driven level loop: one kernel per phase and one small D2H copy of the
 // control block per level; the decision runs on the host from the same
 // device statistics the persistent kernel reads.
@@ -312,17 +332,7 @@ static int bfs_host_loop(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs_
         const int32_t mode = xbfs_choose_mode(a.policy, o.schedule, a.sched_len, pred, &s.hs, L,
                                               m_f, a.m, a.alpha, &htag, &used);
         if (o.time_levels) XB_HIP(hipEventRecord(c->ev_pool[2], c->stream));
-        hipLaunchKernelGGL(k_ph_prep, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L);
-        if (mode != XBFS_MODE_BU && !s.q_valid && !s.need_sweep)
-            hipLaunchKernelGGL(k_ph_compact, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L, 0);
-        if (mode == XBFS_MODE_TD_SF)
-            hipLaunchKernelGGL(k_ph_td_sf, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L,
-                               (int32_t)(p2_next == 0));
-        else if (mode == XBFS_MODE_TD_SS)
-            hipLaunchKernelGGL(k_ph_td_ss, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L);
-        else
-            hipLaunchKernelGGL(k_ph_bu, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L);
-        XB_LAUNCHED();
+        XB_HIP(launch_level(c, a, grid, L, mode, s.q_valid, s.need_sweep, p2_next == 0));
         double lvl_s = sweep_s;
         if (o.time_levels) {
             XB_HIP(hipEventRecord(c->ev_pool[3], c->stream));
@@ -501,3 +511,296 @@ int xbfs_unvisited_list(xbfs_ctx *c, const int32_t *level, int32_t *list, int64_
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// per-level probe (training data and the oracle-best schedule)
+// ---------------------------------------------------------------------------
+static int snap_alloc(xbfs_ctx *c)
+{
+    if (c->s_level) return XBFS_OK;
+    const size_t qb = sizeof(int32_t) * (size_t)c->n * XB_QBUFS * XB_BINS;
+    if (hipMalloc(&c->s_level, sizeof(int32_t) * (size_t)c->n) != hipSuccess ||
+        hipMalloc(&c->s_vis, sizeof(uint64_t) * (size_t)c->nwords) != hipSuccess ||
+        hipMalloc(&c->s_fbm, sizeof(uint64_t) * (size_t)c->fstride * XB_FRINGS) != hipSuccess ||
+        hipMalloc(&c->s_qbuf, qb) != hipSuccess || hipMalloc(&c->s_ctl, sizeof(XbCtl)) != hipSuccess)
+        return XBFS_ERR_NOMEM;
+    return XBFS_OK;
+}
+
+// save = true: live -> snapshot; false: snapshot -> live (device copies on
+// the context stream)
+static hipError_t snap_copy(xbfs_ctx *c, bool save)
+{
+    struct { void *live, *snap; size_t bytes; } parts[] = {
+        {c->d_level, c->s_level, sizeof(int32_t) * (size_t)c->n},
+        {c->d_vis, c->s_vis, sizeof(uint64_t) * (size_t)c->nwords},
+        {c->d_fbm, c->s_fbm, sizeof(uint64_t) * (size_t)c->fstride * XB_FRINGS},
+        {c->d_qbuf, c->s_qbuf, sizeof(int32_t) * (size_t)c->n * XB_QBUFS * XB_BINS},
+        {c->d_ctl, c->s_ctl, sizeof(XbCtl)},
+    };
+    for (auto &pt : parts) {
+        hipError_t e = hipMemcpyAsync(save ? pt.snap : pt.live, save ? pt.live : pt.snap, pt.bytes,
+                                      hipMemcpyDeviceToDevice, c->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+static hipError_t launch_coop_level(xbfs_ctx *c, const XbArgs &a, int32_t L, int32_t mode,
+                                    int32_t q_valid, int32_t need_sweep, int32_t enqueue,
+                                    int32_t extra_sweep)
+{
+    XbArgs args = a;
+    void *kargs[] = {&args, &L, &mode, &q_valid, &need_sweep, &enqueue, &extra_sweep};
+    return hipLaunchCooperativeKernel((const void *)k_coop_level, dim3(c->coop_blocks),
+                                      dim3(XB_PERSIST_BLOCK), kargs, 0, c->stream);
+}
+
+extern "C" int xbfs_bfs_probe(xbfs_ctx *c, int32_t src, const xbfs_opts *opts, int32_t follow_best,
+                              int32_t reps, xbfs_probe_rec *rec, int32_t rec_cap, xbfs_stats *st)
+{
+    if (!c || src < 0 || (int64_t)src >= c->n || reps < 1 || reps > 64) return XBFS_ERR_ARG;
+    if (rec_cap < 0 || (rec_cap > 0 && !rec)) return XBFS_ERR_ARG;
+    xbfs_opts o;
+    int rc = xb_resolve_opts(opts, &o);
+    if (rc) return rc;
+    o.want_parent = 0;
+    XB_HIP(hipSetDevice(c->device));
+    if ((rc = snap_alloc(c)) != XBFS_OK) return rc;
+    if (ensure_events(c, 2) != XBFS_OK) return XBFS_ERR_HIP;
+    if (o.policy == XBFS_POLICY_SCHEDULE)
+        XB_HIP(hipMemcpyAsync(c->d_sched, o.schedule, (size_t)o.schedule_len,
+                              hipMemcpyHostToDevice, c->stream));
+    const XbArgs a = make_args(c, src, o);
+    const int grid = o.grid_dim > 0 ? o.grid_dim : c->num_cu * 8;
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl);
+    hipLaunchKernelGGL(k_ph_init, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a);
+    XB_LAUNCHED();
+    XbLoop s;
+    xb_loop_init(&s);
+    long long pend_n = 0, pend_m = 0, n_f = 0;
+    double t_total = 0;
+    std::vector<float> tr((size_t)reps);
+    for (;;) {
+        const int32_t L = s.L;
+        if (s.need_sweep)
+            hipLaunchKernelGGL(k_ph_compact, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L, 1);
+        XB_HIP(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(XbCtl), hipMemcpyDeviceToHost, c->stream));
+        XB_HIP(hipStreamSynchronize(c->stream));
+        const XbCtl &h = *c->h_ctl;
+        if (h.error) return XBFS_ERR_DEVICE;
+        long long m_f;
+        if (s.need_sweep) {
+            n_f = (long long)h.sw_n[L & 1];
+            m_f = (long long)h.sw_m[L & 1];
+        } else {
+            n_f = (long long)h.slot_n[L & 3];
+            m_f = (long long)h.slot_m[L & 3];
+        }
+        pend_n = (long long)h.slot_n[(L + 1) & 3];
+        pend_m = (long long)h.slot_m[(L + 1) & 3];
+        const long long p2_next = (long long)h.slot_p2[(L + 1) & 3];
+        long long n_u = 0, m_u = 0;
+        if (!xb_level_open(s, a, n_f, m_f, pend_n, pend_m, &n_u, &m_u)) break;
+        if (L >= a.max_levels) return XBFS_ERR_LIMIT;
+        // time every mode from the same state
+        double tm[3];
+        XB_HIP(snap_copy(c, true));
+        for (int m = 0; m < 3; m++) {
+            for (int r = 0; r < reps; r++) {
+                XB_HIP(hipEventRecord(c->ev_pool[0], c->stream));
+                XB_HIP(launch_coop_level(c, a, L, m, s.q_valid, s.need_sweep, p2_next == 0,
+                                         m == XBFS_MODE_TD_SS));
+                XB_HIP(hipEventRecord(c->ev_pool[1], c->stream));
+                XB_HIP(hipEventSynchronize(c->ev_pool[1]));
+                XB_HIP(hipEventElapsedTime(&tr[(size_t)r], c->ev_pool[0], c->ev_pool[1]));
+                XB_HIP(snap_copy(c, false));
+            }
+            std::sort(tr.begin(), tr.end());
+            tm[m] = tr[(size_t)reps / 2] * 1e-3;
+        }
+        float y[XB_MLP_NO];
+        const float *pred = nullptr;
+        if (a.mlp) {
+            float x[XB_MLP_NI];
+            xbfs_mlp_features(L, n_f, m_f, n_u, m_u, s.prev_mode, x);
+            xbfs_mlp_forward1(c->h_mlp.data(), x, y);
+            pred = y;
+        }
+        int32_t htag = 0, used = 0;
+        int32_t mode = xbfs_choose_mode(a.policy, o.schedule, a.sched_len, pred, &s.hs, L, m_f, a.m,
+                                        a.alpha, &htag, &used);
+        if (follow_best) {
+            mode = 0;
+            for (int m = 1; m < 3; m++)
+                if (tm[m] < tm[mode]) mode = m;
+        }
+        XB_HIP(launch_coop_level(c, a, L, mode, s.q_valid, s.need_sweep, p2_next == 0, 0));
+        t_total += tm[mode];
+        if (rec && L < rec_cap) {
+            xbfs_probe_rec &r = rec[L];
+            r.level = L;
+            r.prev_mode = s.prev_mode;
+            r.mode = mode;
+            r.htag = htag;
+            r.n_f = n_f;
+            r.m_f = m_f;
+            r.n_u = n_u;
+            r.m_u = m_u;
+            for (int m = 0; m < 3; m++) r.t_mode[m] = tm[m];
+        }
+        s.q_valid = (mode == XBFS_MODE_TD_SF && p2_next == 0);
+        s.need_sweep = (mode == XBFS_MODE_TD_SS);
+        s.prev_mode = mode;
+        s.L = L + 1;
+    }
+    XB_HIP(hipStreamSynchronize(c->stream));
+    if (st) {
+        const int32_t L = s.L;
+        st->t_bfs_s = t_total; // sum of the executed modes' probed times
+        st->n_levels = L;
+        st->max_level = pend_n > 0 ? L + 1 : (n_f > 0 ? L : L - 1);
+        st->v_cc = s.cum_n + pend_n;
+        st->m_cc = s.cum_m + pend_m;
+        st->td_s = st->bu_s = st->gen_s = 0;
+    }
+    return XBFS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned policy: features, weights, MLP kernels
+// ---------------------------------------------------------------------------
+extern "C" int xbfs_policy_features(int32_t level, int64_t n_f, int64_t m_f, int64_t n_u,
+                                    int64_t m_u, int32_t prev_mode, float *x8)
+{
+    if (!x8 || level < 0) return XBFS_ERR_ARG;
+    xbfs_mlp_features(level, n_f, m_f, n_u, m_u, prev_mode, x8);
+    return XBFS_OK;
+}
+
+extern "C" int xbfs_policy_load(xbfs_ctx *c, const float *w, int32_t nw)
+{
+    if (!c || !w || nw != XB_MLP_NPARAM) return XBFS_ERR_ARG;
+    for (int i = 0; i < nw; i++)
+        if (!(w[i] > -1e30f && w[i] < 1e30f)) return XBFS_ERR_ARG;
+    XB_HIP(hipSetDevice(c->device));
+    XB_HIP(hipMemcpy(c->d_mlp, w, sizeof(float) * XB_MLP_NPARAM, hipMemcpyHostToDevice));
+    c->h_mlp.assign(w, w + XB_MLP_NPARAM);
+    c->mlp_loaded = 1;
+    return XBFS_OK;
+}
+
+namespace {
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    bool alloc(size_t bytes) { return hipMalloc(&p, bytes > 0 ? bytes : 1) == hipSuccess; }
+    template <typename T> T *as() { return (T *)p; }
+};
+constexpr int kMlpGradBlocks = 64;
+}
+
+static int mlp_device(int device)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return XBFS_ERR_HIP;
+    if (device < 0 || device >= ndev) return XBFS_ERR_ARG;
+    return hipSetDevice(device) == hipSuccess ? XBFS_OK : XBFS_ERR_HIP;
+}
+
+extern "C" int xbfs_mlp_forward(int device, const float *params, const float *x, int32_t n,
+                                float *out)
+{
+    if (!params || n < 0 || (n > 0 && (!x || !out))) return XBFS_ERR_ARG;
+    int rc = mlp_device(device);
+    if (rc) return rc;
+    if (n == 0) return XBFS_OK;
+    DevBuf dp, dx, dy;
+    if (!dp.alloc(sizeof(float) * XB_MLP_NPARAM) || !dx.alloc(sizeof(float) * (size_t)n * 8) ||
+        !dy.alloc(sizeof(float) * (size_t)n * 3))
+        return XBFS_ERR_NOMEM;
+    XB_HIP(hipMemcpy(dp.p, params, sizeof(float) * XB_MLP_NPARAM, hipMemcpyHostToDevice));
+    XB_HIP(hipMemcpy(dx.p, x, sizeof(float) * (size_t)n * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mlp_forward, dim3(grid_for(n, XB_MLP_BLOCK, 1024)), dim3(XB_MLP_BLOCK), 0,
+                       0, dp.as<float>(), dx.as<float>(), n, dy.as<float>());
+    XB_LAUNCHED();
+    XB_HIP(hipDeviceSynchronize());
+    XB_HIP(hipMemcpy(out, dy.p, sizeof(float) * (size_t)n * 3, hipMemcpyDeviceToHost));
+    return XBFS_OK;
+}
+
+struct MlpData {
+    DevBuf p, x, y, mk, pg, pl, g, loss;
+};
+
+static int mlp_upload(MlpData &d, const float *params, const float *x, const float *y,
+                      const float *mask, int32_t n)
+{
+    if (!d.p.alloc(sizeof(float) * XB_MLP_NPARAM) || !d.x.alloc(sizeof(float) * (size_t)n * 8) ||
+        !d.y.alloc(sizeof(float) * (size_t)n * 3) || !d.mk.alloc(sizeof(float) * (size_t)n * 3) ||
+        !d.pg.alloc(sizeof(float) * XB_MLP_NPARAM * kMlpGradBlocks) ||
+        !d.pl.alloc(sizeof(float) * kMlpGradBlocks) || !d.g.alloc(sizeof(float) * XB_MLP_NPARAM) ||
+        !d.loss.alloc(sizeof(float)))
+        return XBFS_ERR_NOMEM;
+    if (hipMemcpy(d.p.p, params, sizeof(float) * XB_MLP_NPARAM, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.x.p, x, sizeof(float) * (size_t)n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.y.p, y, sizeof(float) * (size_t)n * 3, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.mk.p, mask, sizeof(float) * (size_t)n * 3, hipMemcpyHostToDevice) != hipSuccess)
+        return XBFS_ERR_HIP;
+    return XBFS_OK;
+}
+
+static void mlp_enqueue_grad(MlpData &d, int32_t n)
+{
+    hipLaunchKernelGGL(k_mlp_grad, dim3(kMlpGradBlocks), dim3(XB_MLP_BLOCK), 0, 0, d.p.as<float>(),
+                       d.x.as<float>(), d.y.as<float>(), d.mk.as<float>(), n, d.pg.as<float>(),
+                       d.pl.as<float>());
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((XB_MLP_NPARAM + 255) / 256), dim3(256), 0, 0,
+                       d.pg.as<float>(), d.pl.as<float>(), kMlpGradBlocks, d.g.as<float>(),
+                       d.loss.as<float>());
+}
+
+extern "C" int xbfs_mlp_loss_grad(int device, const float *params, const float *x, const float *y,
+                                  const float *mask, int32_t n, float *grad, float *loss)
+{
+    if (!params || !x || !y || !mask || n < 1 || !grad || !loss) return XBFS_ERR_ARG;
+    int rc = mlp_device(device);
+    if (rc) return rc;
+    MlpData d;
+    if ((rc = mlp_upload(d, params, x, y, mask, n)) != XBFS_OK) return rc;
+    mlp_enqueue_grad(d, n);
+    XB_LAUNCHED();
+    XB_HIP(hipDeviceSynchronize());
+    XB_HIP(hipMemcpy(grad, d.g.p, sizeof(float) * XB_MLP_NPARAM, hipMemcpyDeviceToHost));
+    XB_HIP(hipMemcpy(loss, d.loss.p, sizeof(float), hipMemcpyDeviceToHost));
+    return XBFS_OK;
+}
+
+extern "C" int xbfs_mlp_train(int device, float *params, const float *x, const float *y,
+                              const float *mask, int32_t n, int32_t steps, float lr,
+                              float *loss_curve)
+{
+    if (!params || !x || !y || !mask || n < 1 || steps < 1 || !(lr > 0.0f)) return XBFS_ERR_ARG;
+    int rc = mlp_device(device);
+    if (rc) return rc;
+    MlpData d;
+    if ((rc = mlp_upload(d, params, x, y, mask, n)) != XBFS_OK) return rc;
+    DevBuf m, v, curve;
+    if (!m.alloc(sizeof(float) * XB_MLP_NPARAM) || !v.alloc(sizeof(float) * XB_MLP_NPARAM) ||
+        !curve.alloc(sizeof(float) * (size_t)steps))
+        return XBFS_ERR_NOMEM;
+    XB_HIP(hipMemset(m.p, 0, sizeof(float) * XB_MLP_NPARAM));
+    XB_HIP(hipMemset(v.p, 0, sizeof(float) * XB_MLP_NPARAM));
+    for (int t = 1; t <= steps; t++) {
+        mlp_enqueue_grad(d, n);
+        hipLaunchKernelGGL(k_mlp_adam, dim3((XB_MLP_NPARAM + 255) / 256), dim3(256), 0, 0,
+                           d.p.as<float>(), d.g.as<float>(), m.as<float>(), v.as<float>(), t, lr,
+                           0.9f, 0.999f, 1e-8f, d.loss.as<float>(), curve.as<float>());
+    }
+    XB_LAUNCHED();
+    XB_HIP(hipDeviceSynchronize());
+    XB_HIP(hipMemcpy(params, d.p.p, sizeof(float) * XB_MLP_NPARAM, hipMemcpyDeviceToHost));
+    if (loss_curve)
+        XB_HIP(hipMemcpy(loss_curve, curve.p, sizeof(float) * (size_t)steps, hipMemcpyDeviceToHost));
+    return XBFS_OK;
+}

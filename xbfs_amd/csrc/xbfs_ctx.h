@@ -43,6 +43,11 @@ struct xbfs_ctx {
     int32_t *d_tile = nullptr;
     int32_t *d_list_cnt = nullptr;
     int64_t ntiles_list = 0;
+    // snapshot of the traversal state for per-level probes (allocated lazily)
+    int32_t *s_level = nullptr;
+    uint64_t *s_vis = nullptr, *s_fbm = nullptr;
+    int32_t *s_qbuf = nullptr;
+    XbCtl *s_ctl = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::vector<hipEvent_t> ev_pool;
 };

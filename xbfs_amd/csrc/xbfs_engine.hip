@@ -655,6 +655,32 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
     }
 }
 
+// One level of the persistent kernel as its own cooperative launch (same
+// block shape, same phases, same barriers): the per-level probe times the
+// three modes with it so its labels match what k_persistent would spend.
+// extra_sweep additionally runs the frontier sweep of level L+1 that a
+// single-scan level leaves to its successor.
+__global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_coop_level(XbArgs a, int32_t L, int32_t mode,
+                                                                 int32_t q_valid, int32_t need_sweep,
+                                                                 int32_t enqueue, int32_t extra_sweep)
+{
+    __shared__ XbSmem sm;
+    const int bid = blockIdx.x, nb = gridDim.x;
+    XbCtl *c = a.ctl;
+    ph_level_prep(a, L, bid, nb);
+    if (mode != XBFS_MODE_BU && !q_valid && !need_sweep) {
+        ph_compact(a, L, bid, nb, false, nullptr, nullptr, sm);
+        if (!xb_grid_sync(c)) return;
+    }
+    if (mode == XBFS_MODE_TD_SF) ph_td<true>(a, L, bid, nb, enqueue != 0, sm);
+    else if (mode == XBFS_MODE_TD_SS) ph_td<false>(a, L, bid, nb, false, sm);
+    else ph_bu(a, L, bid, nb, sm);
+    if (extra_sweep) {
+        if (!xb_grid_sync(c)) return;
+        ph_compact(a, L + 1, bid, nb, true, &c->sw_n[(L + 1) & 1], &c->sw_m[(L + 1) & 1], sm);
+    }
+}
+
 // ---- per-phase kernels for the host-driven loop ---------------------------
 __global__ void __launch_bounds__(XB_PBLOCK) k_ph_init(XbArgs a) { ph_init(a, blockIdx.x, gridDim.x); }
 

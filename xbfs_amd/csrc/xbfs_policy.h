@@ -122,13 +122,15 @@ XBFS_HD inline int32_t xbfs_mode_to_tag(int32_t mode)
 
 XBFS_HD inline float xb_log2p1(long long v)
 {
-    // log2(1+v) without libm: exponent from the leading bit, mantissa by a
-    // cubic fit of log2(1+f) on [0,1) (max error ~1e-3, plenty for features)
+    // log2(1+v) without libm: the exponent from repeated halving, the
+    // mantissa part by a degree-5 least-squares fit of log2(1+f) on [0,1]
+    // that is exact at f = 0 and f = 1 (max error 2.2e-5; features only)
     double x = (double)v + 1.0;
     int e = 0;
     while (x >= 2.0) { x *= 0.5; e++; }
-    double f = x - 1.0;
-    double l = f * (1.4425 - f * (0.7021 - f * 0.2596));
+    const double f = x - 1.0;
+    const double l =
+        f * (1.4417403 + f * (-0.70777018 + f * (0.41234422 + f * (-0.19031903 + f * 0.04400469))));
     return (float)(e + l);
 }
 
