@@ -17,6 +17,10 @@
 #include "xbfs_engine.hip"
 #include "xbfs_mlp.hip"
 
+// dynamic LDS: one XbWaveLds per wave of the block
+#define XB_LDS_PHASE (sizeof(XbWaveLds) * (XB_PBLOCK / 64))
+#define XB_LDS_PERSIST (sizeof(XbWaveLds) * (XB_PERSIST_BLOCK / 64))
+
 #define XB_HIP(expr)                                                                   \
     do {                                                                               \
         hipError_t e_ = (expr);                                                        \
@@ -46,7 +50,7 @@ int xb_ctx_free_device(xbfs_ctx *c)
     void *bufs[] = {c->d_beg,   c->d_adj,  c->d_nz,    c->d_vis,  c->d_fbm,   c->d_level,
                     c->d_parent, c->d_qbuf, c->d_ctl,  c->d_recs, c->d_sched, c->d_mlp,
                     c->d_list,  c->d_ubits, c->d_tile, c->d_list_cnt, c->s_level,
-                    c->s_vis,   c->s_fbm,   c->s_qbuf, c->s_ctl};
+                    c->s_vis,   c->s_fbm,   c->s_qbuf, c->s_ctl, c->d_part, c->s_part};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
@@ -117,11 +121,12 @@ int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, i
         return fail(XBFS_ERR_HIP);
     if (clk_khz > 0) c->wall_hz = (double)clk_khz * 1e3;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent, XB_PERSIST_BLOCK, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent, XB_PERSIST_BLOCK,
+                                                     XB_LDS_PERSIST) !=
             hipSuccess ||
         per_cu < 1)
         return fail(XBFS_ERR_HIP);
-    c->coop_blocks = per_cu * c->num_cu;
+    c->coop_blocks = std::min(per_cu * c->num_cu, XB_PMAX);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(XBFS_ERR_HIP);
     const size_t mm = (size_t)(m > 0 ? m : 1);
@@ -136,6 +141,7 @@ int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, i
               hipMalloc(&c->d_qbuf, sizeof(int32_t) * (size_t)n * XB_QBUFS * XB_BINS) ==
                   hipSuccess &&
               hipMalloc(&c->d_ctl, sizeof(XbCtl)) == hipSuccess &&
+              hipMalloc(&c->d_part, sizeof(long long) * XB_NFIELDS * XB_PMAX) == hipSuccess &&
               hipMalloc(&c->d_recs, sizeof(XbDevRec) * XB_REC_CAP) == hipSuccess &&
               hipMalloc(&c->d_sched, XB_REC_CAP) == hipSuccess &&
               hipMalloc(&c->d_mlp, sizeof(float) * XB_MLP_NPARAM) == hipSuccess &&
@@ -176,7 +182,7 @@ int xb_resolve_opts(const xbfs_opts *in, xbfs_opts *o)
     else xbfs_opts_default(o);
     if (o->block_dim == 0) o->block_dim = XB_PBLOCK;
     if (!pow2_block(o->block_dim)) return XBFS_ERR_ARG;
-    if (o->grid_dim < 0) return XBFS_ERR_ARG;
+    if (o->grid_dim < 0 || o->grid_dim > XB_PMAX) return XBFS_ERR_ARG;
     if (o->bin_thread_max <= 0) o->bin_thread_max = XB_DEFAULT_T1;
     if (o->bin_wave_max <= 0) o->bin_wave_max = XB_DEFAULT_T2;
     if (o->bin_block_max <= 0) o->bin_block_max = std::max<int32_t>(XB_DEFAULT_T3, o->bin_wave_max);
@@ -215,6 +221,7 @@ static XbArgs make_args(xbfs_ctx *c, int32_t src, const xbfs_opts &o)
     a.fbm = c->d_fbm;
     a.qbuf = c->d_qbuf;
     a.ctl = c->d_ctl;
+    a.part = c->d_part;
     a.recs = c->d_recs;
     a.rec_cap = XB_REC_CAP;
     a.sched = c->d_sched;
@@ -260,14 +267,15 @@ static hipError_t launch_level(xbfs_ctx *c, const XbArgs &a, int grid, int32_t L
 {
     hipLaunchKernelGGL(k_ph_prep, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L);
     if (mode != XBFS_MODE_BU && !q_valid && !need_sweep)
-        hipLaunchKernelGGL(k_ph_compact, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L, 0);
+        hipLaunchKernelGGL(k_ph_sweep, dim3(grid), dim3(XB_PBLOCK), XB_LDS_PHASE, c->stream, a, L,
+                           0, 0);
     if (mode == XBFS_MODE_TD_SF)
         hipLaunchKernelGGL(k_ph_td_sf, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L,
                            (int32_t)enqueue);
     else if (mode == XBFS_MODE_TD_SS)
         hipLaunchKernelGGL(k_ph_td_ss, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L);
     else
-        hipLaunchKernelGGL(k_ph_bu, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L);
+        hipLaunchKernelGGL(k_ph_bu, dim3(grid), dim3(XB_PBLOCK), XB_LDS_PHASE, c->stream, a, L);
     return hipGetLastError();
 }
 
@@ -292,10 +300,12 @@ static int bfs_host_loop(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs_
         double sweep_s = 0;
         if (s.need_sweep) {
             if (o.time_levels) XB_HIP(hipEventRecord(c->ev_pool[0], c->stream));
-            hipLaunchKernelGGL(k_ph_compact, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L, 1);
+            hipLaunchKernelGGL(k_ph_sweep, dim3(grid), dim3(XB_PBLOCK), XB_LDS_PHASE, c->stream, a,
+                               L, 1, 1);
             XB_LAUNCHED();
             if (o.time_levels) XB_HIP(hipEventRecord(c->ev_pool[1], c->stream));
         }
+        hipLaunchKernelGGL(k_part_reduce, dim3(1), dim3(1024), 0, c->stream, a);
         XB_HIP(hipMemcpyAsync(c->h_ctl, ctl, sizeof(XbCtl), hipMemcpyDeviceToHost, c->stream));
         XB_HIP(hipStreamSynchronize(c->stream));
         if (o.time_levels && s.need_sweep) {
@@ -307,15 +317,15 @@ static int bfs_host_loop(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs_
         const XbCtl &h = *c->h_ctl;
         long long m_f;
         if (s.need_sweep) {
-            n_f = (long long)h.sw_n[L & 1];
-            m_f = (long long)h.sw_m[L & 1];
+            n_f = h.sum[xb_f_sweep(L, 0)];
+            m_f = h.sum[xb_f_sweep(L, 1)];
         } else {
-            n_f = (long long)h.slot_n[L & 3];
-            m_f = (long long)h.slot_m[L & 3];
+            n_f = h.sum[xb_f_ring(L, 0)];
+            m_f = h.sum[xb_f_ring(L, 1)];
         }
-        pend_n = (long long)h.slot_n[(L + 1) & 3];
-        pend_m = (long long)h.slot_m[(L + 1) & 3];
-        const long long p2_next = (long long)h.slot_p2[(L + 1) & 3];
+        pend_n = h.sum[xb_f_ring(L + 1, 0)];
+        pend_m = h.sum[xb_f_ring(L + 1, 1)];
+        const long long p2_next = h.sum[xb_f_ring(L + 1, 2)];
         long long n_u = 0, m_u = 0;
         if (!xb_level_open(s, a, n_f, m_f, pend_n, pend_m, &n_u, &m_u)) break;
         if (L >= a.max_levels) return XBFS_ERR_LIMIT;
@@ -379,7 +389,7 @@ static int bfs_persistent(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs
     XbArgs args = a;
     void *kargs[] = {&args};
     XB_HIP(hipLaunchCooperativeKernel((const void *)k_persistent, dim3(grid), dim3(XB_PERSIST_BLOCK),
-                                      kargs, 0, c->stream));
+                                      kargs, XB_LDS_PERSIST, c->stream));
     XB_HIP(hipEventRecord(c->ev_stop, c->stream));
     XB_HIP(hipEventSynchronize(c->ev_stop));
     XB_HIP(hipMemcpy(c->h_ctl, c->d_ctl, sizeof(XbCtl), hipMemcpyDeviceToHost));
@@ -522,7 +532,8 @@ static int snap_alloc(xbfs_ctx *c)
     if (hipMalloc(&c->s_level, sizeof(int32_t) * (size_t)c->n) != hipSuccess ||
         hipMalloc(&c->s_vis, sizeof(uint64_t) * (size_t)c->nwords) != hipSuccess ||
         hipMalloc(&c->s_fbm, sizeof(uint64_t) * (size_t)c->fstride * XB_FRINGS) != hipSuccess ||
-        hipMalloc(&c->s_qbuf, qb) != hipSuccess || hipMalloc(&c->s_ctl, sizeof(XbCtl)) != hipSuccess)
+        hipMalloc(&c->s_qbuf, qb) != hipSuccess || hipMalloc(&c->s_ctl, sizeof(XbCtl)) != hipSuccess ||
+        hipMalloc(&c->s_part, sizeof(long long) * XB_NFIELDS * XB_PMAX) != hipSuccess)
         return XBFS_ERR_NOMEM;
     return XBFS_OK;
 }
@@ -537,6 +548,7 @@ static hipError_t snap_copy(xbfs_ctx *c, bool save)
         {c->d_fbm, c->s_fbm, sizeof(uint64_t) * (size_t)c->fstride * XB_FRINGS},
         {c->d_qbuf, c->s_qbuf, sizeof(int32_t) * (size_t)c->n * XB_QBUFS * XB_BINS},
         {c->d_ctl, c->s_ctl, sizeof(XbCtl)},
+        {c->d_part, c->s_part, sizeof(long long) * XB_NFIELDS * XB_PMAX},
     };
     for (auto &pt : parts) {
         hipError_t e = hipMemcpyAsync(save ? pt.snap : pt.live, save ? pt.live : pt.snap, pt.bytes,
@@ -553,7 +565,7 @@ static hipError_t launch_coop_level(xbfs_ctx *c, const XbArgs &a, int32_t L, int
     XbArgs args = a;
     void *kargs[] = {&args, &L, &mode, &q_valid, &need_sweep, &enqueue, &extra_sweep};
     return hipLaunchCooperativeKernel((const void *)k_coop_level, dim3(c->coop_blocks),
-                                      dim3(XB_PERSIST_BLOCK), kargs, 0, c->stream);
+                                      dim3(XB_PERSIST_BLOCK), kargs, XB_LDS_PERSIST, c->stream);
 }
 
 extern "C" int xbfs_bfs_probe(xbfs_ctx *c, int32_t src, const xbfs_opts *opts, int32_t follow_best,
@@ -584,22 +596,24 @@ extern "C" int xbfs_bfs_probe(xbfs_ctx *c, int32_t src, const xbfs_opts *opts, i
     for (;;) {
         const int32_t L = s.L;
         if (s.need_sweep)
-            hipLaunchKernelGGL(k_ph_compact, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a, L, 1);
+            hipLaunchKernelGGL(k_ph_sweep, dim3(grid), dim3(XB_PBLOCK), XB_LDS_PHASE, c->stream, a,
+                               L, 1, 1);
+        hipLaunchKernelGGL(k_part_reduce, dim3(1), dim3(1024), 0, c->stream, a);
         XB_HIP(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(XbCtl), hipMemcpyDeviceToHost, c->stream));
         XB_HIP(hipStreamSynchronize(c->stream));
         const XbCtl &h = *c->h_ctl;
         if (h.error) return XBFS_ERR_DEVICE;
         long long m_f;
         if (s.need_sweep) {
-            n_f = (long long)h.sw_n[L & 1];
-            m_f = (long long)h.sw_m[L & 1];
+            n_f = h.sum[xb_f_sweep(L, 0)];
+            m_f = h.sum[xb_f_sweep(L, 1)];
         } else {
-            n_f = (long long)h.slot_n[L & 3];
-            m_f = (long long)h.slot_m[L & 3];
+            n_f = h.sum[xb_f_ring(L, 0)];
+            m_f = h.sum[xb_f_ring(L, 1)];
         }
-        pend_n = (long long)h.slot_n[(L + 1) & 3];
-        pend_m = (long long)h.slot_m[(L + 1) & 3];
-        const long long p2_next = (long long)h.slot_p2[(L + 1) & 3];
+        pend_n = h.sum[xb_f_ring(L + 1, 0)];
+        pend_m = h.sum[xb_f_ring(L + 1, 1)];
+        const long long p2_next = h.sum[xb_f_ring(L + 1, 2)];
         long long n_u = 0, m_u = 0;
         if (!xb_level_open(s, a, n_f, m_f, pend_n, pend_m, &n_u, &m_u)) break;
         if (L >= a.max_levels) return XBFS_ERR_LIMIT;

@@ -31,6 +31,8 @@ struct xbfs_ctx {
     int32_t *d_parent = nullptr;
     int32_t *d_qbuf = nullptr;  // 3 queue buffers x 4 degree bins x n
     XbCtl *d_ctl = nullptr;
+    long long *d_part = nullptr; // XB_NFIELDS x XB_PMAX partial statistics
+    long long *s_part = nullptr; // probe snapshot of d_part
     XbCtl *h_ctl = nullptr;     // pinned mirror
     XbDevRec *d_recs = nullptr; // XB_REC_CAP records
     int8_t *d_sched = nullptr;  // schedule copy (XB_REC_CAP entries)

@@ -20,6 +20,11 @@
 // scope): MI355X's eight XCDs keep private L2 caches, and a line dirtied by
 // plain stores in one XCD's L2 must never hold words other XCDs update
 // atomically.  The barrier words sit on cache lines of their own.
+struct alignas(128) XbCounter {
+    int32_t v;
+    int32_t pad[31];
+};
+
 struct alignas(256) XbCtl {
     uint32_t bar_count;                 // grid barrier arrivals
     uint32_t bar_pad0[31];
@@ -27,15 +32,22 @@ struct alignas(256) XbCtl {
     uint32_t bar_pad1[31];
     int32_t error;                      // 1 barrier timeout, 2 level cap
     int32_t err_pad[31];
-    int32_t qcnt[XB_QBUFS][XB_BINS];    // queue lengths
-    unsigned long long slot_n[4];       // vertices discovered for a level
-    unsigned long long slot_m[4];       // ... and the sum of their degrees
-    unsigned long long slot_p2[4];      // ... of which set early by the +2 rule
-    unsigned long long sw_n[2], sw_m[2]; // frontier statistics from a bitmap sweep
-    uint32_t tile_ctr[XB_QBUFS];        // bottom-up tile dispenser, ring by level % 3
-    int32_t n_levels, max_level, pad;
+    XbCounter qcnt[XB_QBUFS][XB_BINS];  // queue lengths, one cache line each
+    int32_t n_levels, max_level, pad0, pad1;
     long long v_cc, m_cc;
+    long long sum[16];                  // host-driven loop: reduced statistics (k_part_reduce)
 };
+
+// Per-level statistics live in per-block partial sums (no atomics): block b
+// adds its totals to part[f * XB_PMAX + b] with plain stores (only block b
+// touches column b within a phase) and readers sum the column after the
+// barrier.  Fields: ring slot r = level & 3 -> f = 3r + {0: vertices
+// discovered for that level, 1: their degree sum, 2: of which by the +2 rule};
+// sweep slot s = level & 1 -> f = 12 + 2s + {0: count, 1: degree sum}.
+#define XB_PMAX 4096
+#define XB_NFIELDS 16
+__host__ __device__ inline int xb_f_ring(int32_t level, int k) { return 3 * (level & 3) + k; }
+__host__ __device__ inline int xb_f_sweep(int32_t level, int k) { return 12 + 2 * (level & 1) + k; }
 
 // Per-level record written on the device by the persistent kernel.
 struct XbDevRec {
@@ -54,6 +66,7 @@ struct XbArgs {
     uint64_t *fbm;      // XB_FRINGS bitmaps of nwords: bit v of ring l&3 <=> level[v] == l
     int32_t *qbuf;      // XB_QBUFS x XB_BINS x n
     XbCtl *ctl;
+    long long *part; // XB_NFIELDS x XB_PMAX partial statistics
     XbDevRec *recs;
     int32_t rec_cap;
     const int8_t *sched;
@@ -76,31 +89,25 @@ __host__ __device__ inline uint64_t *xb_f(const XbArgs &a, int32_t level)
     return a.fbm + (int64_t)(level & 3) * a.fstride;
 }
 
-// One LDS buffer shared by all phases (the persistent kernel runs them in
-// sequence, so their scratch can overlap).
+// Block-wide LDS shared by the phases (static).
 union XbSmem {
-    struct {
-        int32_t list[XB_BU_TILE_WORDS * 64];      // candidate vertices of the tile
-        unsigned long long vacc[XB_BU_TILE_WORDS]; // candidate masks, then new visited bits
-        unsigned long long f1acc[XB_BU_TILE_WORDS]; // level L+1 discoveries
-        unsigned long long f2acc[XB_BU_TILE_WORDS]; // +2 discoveries
-        int woff[XB_BU_TILE_WORDS + 1];            // exclusive prefix of candidates per word
-        uint32_t tile;
-    } bu;
     struct {
         long long pref[XB_GCAP + 1]; // exclusive degree prefix of grid-bin vertices
         int32_t u[XB_GCAP];
     } td;
     struct {
-        int32_t list[XB_BU_TILE_WORDS * 64];       // frontier vertices of the tile
-        unsigned long long words[XB_BU_TILE_WORDS];
-        int woff[XB_BU_TILE_WORDS + 1];
-        int wcnt[16 * XB_BINS];                    // per-wave bin counts, then offsets
-        int base[XB_BINS];                         // reserved queue positions
-    } cp;
-    struct {
         float x[8], h1[32], h2[32], y[4];
     } mlp;
+};
+
+#define XB_CHUNK_WORDS 16 /* bitmap words per wave-owned chunk: 1024 vertices, one 128-B line */
+
+// Per-wave LDS region (dynamic shared memory, one per wave of the block).
+struct XbWaveLds {
+    int32_t list[XB_CHUNK_WORDS * 64];      // the chunk's vertices to process
+    unsigned long long f1[XB_CHUNK_WORDS];  // bottom-up: new level-(L+1) bits
+    unsigned long long f2[XB_CHUNK_WORDS];  // bottom-up: new level-(L+2) bits (+2 rule)
+    unsigned long long va[XB_CHUNK_WORDS];  // bottom-up: new visited bits
 };
 
 // Loop state every block (or the host) tracks identically.

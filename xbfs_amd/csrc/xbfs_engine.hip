@@ -11,16 +11,18 @@
 // Phases (device functions run by every block of a grid, with a grid-wide
 // barrier between phases):
 //   ph_init      level/parent = -1, bitmaps cleared, source seeded
-//   ph_compact   F_L -> degree-binned queue (+ frontier statistics), by
-//                64-bit ballots over bitmap words: the single-scan frontier
-//                generator (bfs_single_scan.cuh:26-50) and the post-bottom-up
-//                queue rebuild ("last phase", bfs_TD_scan_free.cuh:23-135)
+//   ph_sweep     level-L frontier -> degree-binned queue (+ statistics),
+//                wave-centric over 1024-vertex chunks of the frontier bitmap
+//                (or of the level array after a single-scan level): the
+//                single-scan frontier generator (bfs_single_scan.cuh:26-50)
+//                and the post-bottom-up queue rebuild ("last phase",
+//                bfs_TD_scan_free.cuh:23-135)
 //   ph_td<SF>    top-down expansion of the binned queue: SF claims with CAS
 //                and appends winners to the next queue's degree bins
 //                (bfs_adaptiveFQ_wb_async.cuh:50-162); SS uses plain stores and
 //                no queue (bfs_single_scan.cuh:52-126)
 //   ph_bu        bottom-up over unvisited vertices (nz & ~visited), compacted
-//                per 4096-vertex tile in LDS by ballot/popcount and a wave scan
+//                per wave-owned 1024-vertex chunk in LDS by popcount and a wave scan
 //                (ballot_counting/FQ_gen/prefix sum, bfs_adaptiveFQ_wb_async.cuh:
 //                189-284, prefix_sum1.cuh) and updated with the asynchronous
 //                +2 rule (thread_centric_BU_update, bfs_adaptiveFQ_wb_async.cuh:738-777)
@@ -46,6 +48,57 @@ template <typename T>
 __device__ __forceinline__ void xb_st(T *p, T v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Adds the block's totals of K counters to column `bid` of the partial
+// statistics (thread 0, plain read-modify-write: within a phase only this
+// block writes column bid).  Every thread of the block must call it.
+template <int K>
+__device__ void xb_part_add(const XbArgs &a, int bid, unsigned long long (&v)[K], const int (&f)[K])
+{
+    __shared__ unsigned long long sh[K][16];
+    const int w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const unsigned long long t = xb_wave_sum_u64(v[k]);
+        if (xb_lane() == 0) sh[k][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            unsigned long long t = 0;
+            for (int i = 0; i < nw; i++) t += sh[k][i];
+            if (t) a.part[(int64_t)f[k] * XB_PMAX + bid] += (long long)t;
+        }
+    }
+    __syncthreads();
+}
+
+// Block-cooperative sums of K fields over all XB_PMAX columns; every thread
+// of the block gets the results.
+template <int K>
+__device__ void xb_part_sum(const XbArgs &a, const int (&f)[K], long long (&out)[K])
+{
+    __shared__ long long sh[K][16];
+    const int w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+    long long acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        acc[k] = 0;
+        for (int b = threadIdx.x; b < XB_PMAX; b += blockDim.x)
+            acc[k] += a.part[(int64_t)f[k] * XB_PMAX + b];
+        const long long t = (long long)xb_wave_sum_u64((unsigned long long)acc[k]);
+        if (xb_lane() == 0) sh[k][w] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        long long t = 0;
+        for (int i = 0; i < nw; i++) t += sh[k][i];
+        out[k] = t;
+    }
+    __syncthreads();
 }
 
 // Grid-wide barrier for a cooperative launch (all blocks co-resident).  The
@@ -128,58 +181,48 @@ __device__ void ph_init(const XbArgs &a, int bid, int nb)
         xb_f(a, 2)[w] = 0ull;
         xb_f(a, 3)[w] = 0ull;
     }
+    // partial statistics: all zero except level 0's slot = (1, deg(src))
+    const long long dsrc = a.beg[s + 1] - a.beg[s];
+    for (int64_t i = gtid; i < (int64_t)XB_NFIELDS * XB_PMAX; i += nthr) {
+        long long v = 0;
+        if (i == (int64_t)xb_f_ring(0, 0) * XB_PMAX) v = 1;
+        else if (i == (int64_t)xb_f_ring(0, 1) * XB_PMAX) v = dsrc;
+        a.part[i] = v;
+    }
     if (bid == 0 && threadIdx.x == 0) {
         XbCtl *c = a.ctl;
         for (int b = 0; b < XB_QBUFS; b++)
-            for (int k = 0; k < XB_BINS; k++) xb_st(&c->qcnt[b][k], 0);
-        for (int i = 0; i < 4; i++) {
-            xb_st(&c->slot_n[i], 0ull);
-            xb_st(&c->slot_m[i], 0ull);
-            xb_st(&c->slot_p2[i], 0ull);
-        }
-        for (int i = 0; i < 2; i++) {
-            xb_st(&c->sw_n[i], 0ull);
-            xb_st(&c->sw_m[i], 0ull);
-        }
-        for (int i = 0; i < XB_QBUFS; i++) xb_st(&c->tile_ctr[i], 0u);
+            for (int k = 0; k < XB_BINS; k++) xb_st(&c->qcnt[b][k].v, 0);
         xb_st(&c->n_levels, 0);
         xb_st(&c->max_level, 0);
         xb_st(&c->v_cc, 0ll);
         xb_st(&c->m_cc, 0ll);
-        const int64_t d = a.beg[s + 1] - a.beg[s];
-        int bin = d <= a.t1 ? 0 : (d <= a.t2 ? 1 : (d <= a.t3 ? 2 : 3));
-        if (d > 0) {
+        const int bin = dsrc <= a.t1 ? 0 : (dsrc <= a.t2 ? 1 : (dsrc <= a.t3 ? 2 : 3));
+        if (dsrc > 0) {
             xb_q(a, 0, bin)[0] = s;
-            xb_st(&c->qcnt[0][bin], 1);
+            xb_st(&c->qcnt[0][bin].v, 1);
         }
-        xb_st(&c->slot_n[0], 1ull);
-        xb_st(&c->slot_m[0], (unsigned long long)d);
     }
 }
 
-// Zeroing schedule, run by block 0 at the start of level L (before any of
-// level L's writes; everything zeroed here is first written at level L+1):
-//   ring slots (L+3)&3, queue buffer (L+2)%3, sweep slot (L+1)&1,
-//   tile dispenser (L+1)%3.
-// All blocks clear their share of bitmap ring (L+3)&3 = (L-1)&3, which no
-// phase of level L reads or writes and level L+1 first writes.
+// Zeroing schedule at the start of level L (everything zeroed here is
+// first written at level L+1): queue buffer (L+2)%3 (block 0), the partial
+// statistics of ring slot (L+3)&3 and sweep slot (L+1)&1, and bitmap ring
+// (L+3)&3 = (L-1)&3, which no phase of level L reads or writes.
 __device__ void ph_level_prep(const XbArgs &a, int32_t L, int bid, int nb)
 {
     if (bid == 0 && threadIdx.x == 0) {
         XbCtl *c = a.ctl;
-        const int s3 = (L + 3) & 3;
-        xb_st(&c->slot_n[s3], 0ull);
-        xb_st(&c->slot_m[s3], 0ull);
-        xb_st(&c->slot_p2[s3], 0ull);
         const int qb = (L + 2) % XB_QBUFS;
-        for (int k = 0; k < XB_BINS; k++) xb_st(&c->qcnt[qb][k], 0);
-        xb_st(&c->sw_n[(L + 1) & 1], 0ull);
-        xb_st(&c->sw_m[(L + 1) & 1], 0ull);
-        xb_st(&c->tile_ctr[(L + 1) % XB_QBUFS], 0u);
+        for (int k = 0; k < XB_BINS; k++) xb_st(&c->qcnt[qb][k].v, 0);
     }
-    uint64_t *F3 = xb_f(a, L + 3);
     const int64_t gtid = (int64_t)bid * blockDim.x + threadIdx.x;
     const int64_t nthr = (int64_t)nb * blockDim.x;
+    const int zf[5] = {xb_f_ring(L + 3, 0), xb_f_ring(L + 3, 1), xb_f_ring(L + 3, 2),
+                       xb_f_sweep(L + 1, 0), xb_f_sweep(L + 1, 1)};
+    for (int64_t i = gtid; i < 5 * XB_PMAX; i += nthr)
+        a.part[(int64_t)zf[i / XB_PMAX] * XB_PMAX + (i % XB_PMAX)] = 0;
+    uint64_t *F3 = xb_f(a, L + 3);
     for (int64_t w = gtid; w < a.nwords; w += nthr) F3[w] = 0ull;
 }
 
@@ -191,99 +234,152 @@ __device__ __forceinline__ int xb_bin_of(const XbArgs &a, int64_t d)
     return d <= a.t1 ? 0 : (d <= a.t2 ? 1 : (d <= a.t3 ? 2 : 3));
 }
 
-// Block-aggregated append to four degree bins: every thread of the block
-// calls it (bin -1 = nothing to append); one global atomic per non-empty bin.
-__device__ void xb_block_append4(int bin, int32_t val, int32_t *const (&q)[XB_BINS], int32_t *cnt,
-                                 int *s_wcnt, int *s_base)
+// LDS ordering between lanes of one wave: a code-motion barrier plus a
+// wavefront-scope fence (waits for this wave's LDS operations).
+__device__ __forceinline__ void xb_wave_sync()
 {
-    const int w = threadIdx.x >> 6, ln = xb_lane(), nw = (int)(blockDim.x >> 6);
-    unsigned long long m[XB_BINS];
-#pragma unroll
-    for (int k = 0; k < XB_BINS; k++) {
-        m[k] = __ballot(bin == k);
-        if (ln == 0) s_wcnt[w * XB_BINS + k] = (int)__popcll(m[k]);
-    }
-    __syncthreads();
-    if (threadIdx.x < XB_BINS) {
-        const int k = threadIdx.x;
-        int run = 0;
-        for (int i = 0; i < nw; i++) {
-            const int c = s_wcnt[i * XB_BINS + k];
-            s_wcnt[i * XB_BINS + k] = run;
-            run += c;
-        }
-        s_base[k] = run ? atomicAdd(&cnt[k], run) : 0;
-    }
-    __syncthreads();
-    if (bin >= 0) {
-        unsigned long long mk = 0;
-#pragma unroll
-        for (int k = 0; k < XB_BINS; k++)
-            if (k == bin) mk = m[k];
-        q[bin][s_base[bin] + s_wcnt[w * XB_BINS + bin] + xb_rank(mk)] = val;
-    }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
-// F_L -> binned queue of buffer L%3.  Tiles of XB_BU_TILE_WORDS bitmap words
-// (static round robin over blocks): wave 0 reads the words and scans their
-// popcounts, the set bits are expanded into an ascending LDS list, and the
-// list is appended to the bins with one atomic per bin per round.
-__device__ void ph_compact(const XbArgs &a, int32_t L, int bid, int nb, bool stats,
-                           unsigned long long *sn, unsigned long long *sm_, XbSmem &sm)
+// Fills the wave's LDS list with the set bits of the chunk's words (lane k
+// < XB_CHUNK_WORDS holds word w0 + k), in ascending vertex order; returns the
+// count (wave-uniform).
+__device__ __forceinline__ int xb_chunk_list(uint64_t myword, int64_t w0, XbWaveLds *wl)
 {
-    const uint64_t *F = xb_f(a, L);
+    const int ln = xb_lane();
+    const int pc = (int)__popcll(myword);
+    const int inc = xb_wave_incl_scan_i32(pc);
+    const int total = __shfl(inc, 63);
+    if (total == 0) return 0;
+    const int off = inc - pc;
+    const uint64_t below = ln ? (~0ull >> (64 - ln)) : 0ull;
+    for (int k = 0; k < XB_CHUNK_WORDS; k++) {
+        const uint64_t word = __shfl(myword, k);
+        const int o = __shfl(off, k);
+        if ((word >> ln) & 1ull) wl->list[o + (int)__popcll(word & below)] = (int32_t)((w0 + k) * 64 + ln);
+    }
+    xb_wave_sync();
+    return total;
+}
+
+// Frontier sweep of level L, wave-centric: each wave owns chunks of
+// XB_CHUNK_WORDS bitmap words (1024 vertices).
+//   from_levels = false: the level-L vertices are the bits of F_L (complete
+//     whenever the previous level ran bottom-up or scan-free top-down);
+//   from_levels = true: after a single-scan level, whose plain stores set no
+//     bits, the chunk's level words are read (64 per ballot) and the
+//     frontier bits of F_L and the visited bits are written here.
+// Appends the level-L vertices of degree > 0 to their degree bins in queue
+// buffer L%3 (single-scan frontier generation, bfs_single_scan.cuh:26-50,
+// and the post-bottom-up queue rebuild, bfs_TD_scan_free.cuh:23-135).  Two
+// passes: count per wave, reserve once per block and bin (one atomic), then
+// write; the second pass re-reads F_L and the degrees (mostly from L2).
+// With stats, adds (count, degree sum) to sweep slot L&1.
+__device__ void ph_sweep(const XbArgs &a, int32_t L, int bid, int nb, bool from_levels, bool stats,
+                         XbWaveLds *lds)
+{
+    __shared__ int s_wc[16][XB_BINS];
+    __shared__ int s_base[XB_BINS];
+    uint64_t *F = xb_f(a, L);
     const int qb = L % XB_QBUFS;
     int32_t *const q[XB_BINS] = {xb_q(a, qb, 0), xb_q(a, qb, 1), xb_q(a, qb, 2), xb_q(a, qb, 3)};
-    int32_t *cnt = a.ctl->qcnt[qb];
+    XbCounter *gcnt = a.ctl->qcnt[qb];
     const int ln = xb_lane(), wv = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+    XbWaveLds *wl = lds + wv;
+    const int64_t gw = ((int64_t)bid * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwv = ((int64_t)nb * blockDim.x) >> 6;
+    const int64_t nchunks = (a.nwords + XB_CHUNK_WORDS - 1) / XB_CHUNK_WORDS;
     unsigned long long my_n = 0, my_m = 0;
-    for (int64_t tile = bid; tile < a.ntiles; tile += nb) {
-        const int64_t w0 = tile * XB_BU_TILE_WORDS;
-        if (wv == 0) {
-            const int64_t w = w0 + ln;
-            uint64_t word = 0;
-            if (ln < XB_BU_TILE_WORDS && w < a.nwords) word = F[w];
-            const int pc = (int)__popcll(word);
-            const int inc = xb_wave_incl_scan_i32(pc);
-            if (ln < XB_BU_TILE_WORDS) {
-                sm.cp.woff[ln] = inc - pc;
-                sm.cp.words[ln] = word;
+    int wc[XB_BINS] = {0, 0, 0, 0};
+    // pass 1: bits (from the levels when asked), statistics, bin counts
+    for (int64_t ch = gw; ch < nchunks; ch += nwv) {
+        const int64_t w0 = ch * XB_CHUNK_WORDS;
+        const int64_t w = w0 + ln;
+        uint64_t myword = 0;
+        if (from_levels) {
+            for (int k0 = 0; k0 < XB_CHUNK_WORDS; k0 += 8) {
+                int32_t lv[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int64_t v = (w0 + k0 + k) * 64 + ln;
+                    lv[k] = v < a.n ? a.level[v] : -2;
+                }
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const unsigned long long f = __ballot(lv[k] == L);
+                    if (ln == k0 + k) myword = f;
+                }
             }
-            if (ln == 63) sm.cp.woff[XB_BU_TILE_WORDS] = inc;
-        }
-        __syncthreads();
-        const int tcnt = sm.cp.woff[XB_BU_TILE_WORDS];
-        if (tcnt == 0) {
-            __syncthreads();
-            continue;
-        }
-        for (int k = wv; k < XB_BU_TILE_WORDS; k += nw) {
-            const uint64_t word = sm.cp.words[k];
-            if ((word >> ln) & 1ull) {
-                const uint64_t below = ln ? (~0ull >> (64 - ln)) : 0ull;
-                sm.cp.list[sm.cp.woff[k] + (int)__popcll(word & below)] = (int32_t)((w0 + k) * 64 + ln);
+            if (ln < XB_CHUNK_WORDS && w < a.nwords) {
+                F[w] = myword;
+                if (myword) a.vis[w] |= myword;
             }
+        } else if (ln < XB_CHUNK_WORDS && w < a.nwords) {
+            myword = F[w];
         }
-        __syncthreads();
-        for (int r = 0; r < tcnt; r += blockDim.x) {
-            const int i = r + threadIdx.x;
+        const int total = xb_chunk_list(myword, w0, wl);
+        for (int r = 0; r < total; r += 64) {
+            const int i = r + ln;
             int bin = -1;
-            int32_t v = 0;
-            if (i < tcnt) {
-                v = sm.cp.list[i];
+            if (i < total) {
+                const int32_t v = wl->list[i];
                 const int64_t d = a.beg[v + 1] - a.beg[v];
                 my_n += 1;
                 my_m += (unsigned long long)d;
                 if (d > 0) bin = xb_bin_of(a, d);
             }
-            xb_block_append4(bin, v, q, cnt, sm.cp.wcnt, sm.cp.base);
+#pragma unroll
+            for (int k = 0; k < XB_BINS; k++) wc[k] += (int)__popcll(__ballot(bin == k));
         }
+        xb_wave_sync();
+    }
+    // reserve: one atomic per bin per block
+    if (ln == 0)
+        for (int k = 0; k < XB_BINS; k++) s_wc[wv][k] = wc[k];
+    __syncthreads();
+    if (threadIdx.x < XB_BINS) {
+        const int k = threadIdx.x;
+        int run = 0;
+        for (int i = 0; i < nw; i++) {
+            const int c = s_wc[i][k];
+            s_wc[i][k] = run;
+            run += c;
+        }
+        s_base[k] = run ? atomicAdd(&gcnt[k].v, run) : 0;
+    }
+    __syncthreads();
+    int pos[XB_BINS];
+    for (int k = 0; k < XB_BINS; k++) pos[k] = s_base[k] + s_wc[wv][k];
+    // pass 2: write the queue entries
+    for (int64_t ch = gw; ch < nchunks; ch += nwv) {
+        const int64_t w0 = ch * XB_CHUNK_WORDS;
+        const int64_t w = w0 + ln;
+        uint64_t myword = 0;
+        if (ln < XB_CHUNK_WORDS && w < a.nwords) myword = F[w];
+        const int total = xb_chunk_list(myword, w0, wl);
+        for (int r = 0; r < total; r += 64) {
+            const int i = r + ln;
+            int bin = -1;
+            int32_t v = 0;
+            if (i < total) {
+                v = wl->list[i];
+                const int64_t d = a.beg[v + 1] - a.beg[v];
+                if (d > 0) bin = xb_bin_of(a, d);
+            }
+#pragma unroll
+            for (int k = 0; k < XB_BINS; k++) {
+                const unsigned long long m = __ballot(bin == k);
+                if (bin == k) q[k][pos[k] + xb_rank(m)] = v;
+                pos[k] += (int)__popcll(m);
+            }
+        }
+        xb_wave_sync();
     }
     if (stats) {
         unsigned long long vv[2] = {my_n, my_m};
-        unsigned long long *const dd[2] = {sn, sm_};
-        xb_block_add_u64<2>(vv, dd);
+        const int ff[2] = {xb_f_sweep(L, 0), xb_f_sweep(L, 1)};
+        xb_part_add<2>(a, bid, vv, ff);
     }
 }
 
@@ -294,7 +390,7 @@ struct XbTdCtx {
     const int32_t *qc[XB_BINS];
     int32_t nc[XB_BINS];
     int32_t *qn[XB_BINS];
-    int32_t *cn;
+    XbCounter *cn;
     uint64_t *Fn; // bitmap of level L+1
     int32_t L1;
     bool enqueue;
@@ -316,10 +412,12 @@ __device__ __forceinline__ void td_visit(const XbArgs &a, const XbTdCtx &t, int3
         }
         if (won) {
             if (a.parent) a.parent[v] = u;
-            const uint64_t bit = 1ull << (v & 63);
-            atomicOr((unsigned long long *)&a.vis[v >> 6], (unsigned long long)bit);
-            atomicOr((unsigned long long *)&t.Fn[v >> 6], (unsigned long long)bit);
             if (SF) {
+                // single-scan sets no bits: the sweep that must follow it
+                // rebuilds F_{L+1} and the visited bits from the levels
+                const uint64_t bit = 1ull << (v & 63);
+                atomicOr((unsigned long long *)&a.vis[v >> 6], (unsigned long long)bit);
+                atomicOr((unsigned long long *)&t.Fn[v >> 6], (unsigned long long)bit);
                 d = a.beg[v + 1] - a.beg[v];
                 my_n += 1;
                 my_m += (unsigned long long)d;
@@ -328,10 +426,10 @@ __device__ __forceinline__ void td_visit(const XbArgs &a, const XbTdCtx &t, int3
     }
     if (SF && t.enqueue) {
         const int b = won && d > 0 ? xb_bin_of(a, d) : -1;
-        xb_wave_append(t.qn[0], &t.cn[0], b == 0, v);
-        xb_wave_append(t.qn[1], &t.cn[1], b == 1, v);
-        xb_wave_append(t.qn[2], &t.cn[2], b == 2, v);
-        xb_wave_append(t.qn[3], &t.cn[3], b == 3, v);
+        xb_wave_append(t.qn[0], &t.cn[0].v, b == 0, v);
+        xb_wave_append(t.qn[1], &t.cn[1].v, b == 1, v);
+        xb_wave_append(t.qn[2], &t.cn[2].v, b == 2, v);
+        xb_wave_append(t.qn[3], &t.cn[3].v, b == 3, v);
     }
 }
 
@@ -344,7 +442,7 @@ __device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue,
     const int qc = L % XB_QBUFS, qn = (L + 1) % XB_QBUFS;
     for (int k = 0; k < XB_BINS; k++) {
         t.qc[k] = xb_q(a, qc, k);
-        t.nc[k] = xb_ld_i32(&a.ctl->qcnt[qc][k]);
+        t.nc[k] = xb_ld_i32(&a.ctl->qcnt[qc][k].v);
         t.qn[k] = xb_q(a, qn, k);
     }
     t.cn = a.ctl->qcnt[qn];
@@ -416,118 +514,104 @@ __device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue,
     }
     if (SF) {
         unsigned long long vv[2] = {my_n, my_m};
-        unsigned long long *const dd[2] = {&a.ctl->slot_n[(L + 1) & 3], &a.ctl->slot_m[(L + 1) & 3]};
-        xb_block_add_u64<2>(vv, dd);
+        const int ff[2] = {xb_f_ring(L + 1, 0), xb_f_ring(L + 1, 1)};
+        xb_part_add<2>(a, bid, vv, ff);
     }
 }
 
 // ---------------------------------------------------------------------------
 // bottom-up
 // ---------------------------------------------------------------------------
-__device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbSmem &sm)
+__device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *lds)
 {
-    int32_t *s_list = sm.bu.list;
-    unsigned long long *s_vacc = sm.bu.vacc;
-    unsigned long long *s_f1acc = sm.bu.f1acc;
-    unsigned long long *s_f2acc = sm.bu.f2acc;
-    int *s_woff = sm.bu.woff;
-    uint32_t &s_tile = sm.bu.tile;
-    (void)bid;
-    (void)nb;
     const uint64_t *F = xb_f(a, L);
     uint64_t *F1 = xb_f(a, L + 1);
     uint64_t *F2 = xb_f(a, L + 2);
     const int32_t L1 = L + 1, L2 = L + 2;
-    uint32_t *ctr = &a.ctl->tile_ctr[L % XB_QBUFS];
-    const int ln = xb_lane(), wv = threadIdx.x >> 6;
+    const int ln = xb_lane();
+    XbWaveLds *wl = lds + (threadIdx.x >> 6);
+    const int64_t gw = ((int64_t)bid * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwv = ((int64_t)nb * blockDim.x) >> 6;
+    const int64_t nchunks = (a.nwords + XB_CHUNK_WORDS - 1) / XB_CHUNK_WORDS;
+    const uint64_t below = ln ? (~0ull >> (64 - ln)) : 0ull;
     unsigned long long n1 = 0, m1 = 0, n2 = 0, m2 = 0;
-    for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(ctr, 1u);
-        __syncthreads();
-        const int64_t tile = s_tile;
-        if (tile >= a.ntiles) break;
-        const int64_t w0 = tile * XB_BU_TILE_WORDS;
-        // candidate masks and their prefix (wave 0: one word per lane)
-        if (wv == 0) {
-            const int64_t w = w0 + ln;
-            uint64_t c = 0;
-            if (ln < XB_BU_TILE_WORDS && w < a.nwords) c = a.nz[w] & ~a.vis[w];
-            const int pc = (int)__popcll(c);
-            const int inc = xb_wave_incl_scan_i32(pc);
-            if (ln < XB_BU_TILE_WORDS) {
-                s_woff[ln] = inc - pc;
-                s_vacc[ln] = c; // candidate mask, reused below
-                s_f1acc[ln] = 0ull;
-                s_f2acc[ln] = 0ull;
-            }
-            if (ln == 63) s_woff[XB_BU_TILE_WORDS] = inc;
+    for (int64_t ch = gw; ch < nchunks; ch += nwv) {
+        const int64_t w0 = ch * XB_CHUNK_WORDS;
+        const int64_t w = w0 + ln;
+        uint64_t cand = 0; // lane k < 16: unvisited non-isolated vertices of word w0 + k
+        if (ln < XB_CHUNK_WORDS && w < a.nwords) cand = a.nz[w] & ~a.vis[w];
+        const int pc = (int)__popcll(cand);
+        const int inc = xb_wave_incl_scan_i32(pc);
+        const int total = __shfl(inc, 63);
+        if (total == 0) continue;
+        const int off = inc - pc;
+        // the chunk's candidates in ascending order
+        for (int k = 0; k < XB_CHUNK_WORDS; k++) {
+            const uint64_t word = __shfl(cand, k);
+            const int o = __shfl(off, k);
+            if ((word >> ln) & 1ull)
+                wl->list[o + (int)__popcll(word & below)] = (int32_t)((w0 + k) * 64 + ln);
         }
-        __syncthreads();
-        const int cnt = s_woff[XB_BU_TILE_WORDS];
-        // scatter candidate ids in ascending order: wave w handles words w, w+4, ...
-        for (int k = wv; k < XB_BU_TILE_WORDS; k += (int)(blockDim.x >> 6)) {
-            const uint64_t c = s_vacc[k];
-            if ((c >> ln) & 1ull) {
-                const uint64_t below = ln ? (~0ull >> (64 - ln)) : 0ull;
-                s_list[s_woff[k] + (int)__popcll(c & below)] = (int32_t)((w0 + k) * 64 + ln);
-            }
+        if (ln < XB_CHUNK_WORDS) {
+            wl->f1[ln] = 0ull;
+            wl->f2[ln] = 0ull;
+            wl->va[ln] = 0ull;
         }
-        __syncthreads();
-        if (threadIdx.x < XB_BU_TILE_WORDS) s_vacc[threadIdx.x] = 0ull;
-        __syncthreads();
-        for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
-            const int32_t v = s_list[k];
-            const int64_t b = a.beg[v], e = a.beg[v + 1];
-            int32_t found = -1, seen = -1;
-            for (int64_t j = b; j < e; j++) {
-                const int32_t u = a.adj[j];
-                const uint64_t fw = F[u >> 6];
-                if ((fw >> (u & 63)) & 1ull) { found = u; break; }
-                if (seen < 0) {
-                    const uint64_t nw = F1[u >> 6];
-                    if ((nw >> (u & 63)) & 1ull) seen = u;
+        xb_wave_sync();
+        for (int r = 0; r < total; r += 64) {
+            const int i = r + ln;
+            if (i < total) {
+                const int32_t v = wl->list[i];
+                const int64_t b = a.beg[v], e = a.beg[v + 1];
+                int32_t found = -1, seen = -1;
+                for (int64_t j = b; j < e; j++) {
+                    const int32_t u = a.adj[j];
+                    const uint64_t fw = F[u >> 6];
+                    if ((fw >> (u & 63)) & 1ull) { found = u; break; }
+                    if (seen < 0) {
+                        const uint64_t nw = F1[u >> 6];
+                        if ((nw >> (u & 63)) & 1ull) seen = u;
+                    }
+                }
+                const unsigned long long bit = 1ull << (v & 63);
+                const int li = (int)((v >> 6) - w0);
+                if (found >= 0) {
+                    a.level[v] = L1;
+                    if (a.parent) a.parent[v] = found;
+                    atomicOr(&wl->f1[li], bit);
+                    atomicOr(&wl->va[li], bit);
+                    n1 += 1;
+                    m1 += (unsigned long long)(e - b);
+                } else if (seen >= 0) {
+                    a.level[v] = L2;
+                    if (a.parent) a.parent[v] = seen;
+                    atomicOr(&wl->f2[li], bit);
+                    atomicOr(&wl->va[li], bit);
+                    n2 += 1;
+                    m2 += (unsigned long long)(e - b);
                 }
             }
-            const unsigned long long bit = 1ull << (v & 63);
-            const int li = (int)((v >> 6) - w0);
-            if (found >= 0) {
-                a.level[v] = L1;
-                if (a.parent) a.parent[v] = found;
-                atomicOr(&s_f1acc[li], bit);
-                atomicOr(&s_vacc[li], bit);
-                n1 += 1;
-                m1 += (unsigned long long)(e - b);
-            } else if (seen >= 0) {
-                a.level[v] = L2;
-                if (a.parent) a.parent[v] = seen;
-                atomicOr(&s_f2acc[li], bit);
-                atomicOr(&s_vacc[li], bit);
-                n2 += 1;
-                m2 += (unsigned long long)(e - b);
-            }
         }
-        __syncthreads();
-        // this block owns the tile's words for the whole level: plain stores
-        if (threadIdx.x < XB_BU_TILE_WORDS) {
-            const int64_t w = w0 + threadIdx.x;
-            if (w < a.nwords) {
-                const unsigned long long va = s_vacc[threadIdx.x];
-                if (va) a.vis[w] |= va;
-                const unsigned long long f1 = s_f1acc[threadIdx.x];
-                if (f1) F1[w] |= f1; // other tiles' +2 probes see it from here on
-                const unsigned long long f2 = s_f2acc[threadIdx.x];
+        xb_wave_sync();
+        // the wave owns the chunk's words (one cache line per bitmap) for
+        // the whole level: plain read-modify-write stores
+        if (ln < XB_CHUNK_WORDS && w < a.nwords) {
+            const unsigned long long va = wl->va[ln];
+            if (va) {
+                a.vis[w] |= va;
+                const unsigned long long f1 = wl->f1[ln];
+                if (f1) F1[w] |= f1; // other chunks' +2 probes see it from here on
+                const unsigned long long f2 = wl->f2[ln];
                 if (f2) F2[w] |= f2;
             }
         }
-        __syncthreads();
     }
-    // n2 also goes to slot_p2: a top-down level L+1 must not trust its winner
+    // n2 also goes to the +2 field: a top-down level L+1 must not trust its winner
     // queue as the level-(L+2) frontier when +2 discoveries exist
     unsigned long long vv[5] = {n1, m1, n2, m2, n2};
-    unsigned long long *const dd[5] = {&a.ctl->slot_n[L1 & 3], &a.ctl->slot_m[L1 & 3],
-                                       &a.ctl->slot_n[L2 & 3], &a.ctl->slot_m[L2 & 3],
-                                       &a.ctl->slot_p2[L2 & 3]};
-    xb_block_add_u64<5>(vv, dd);
+    const int ff[5] = {xb_f_ring(L1, 0), xb_f_ring(L1, 1), xb_f_ring(L2, 0), xb_f_ring(L2, 1),
+                       xb_f_ring(L2, 2)};
+    xb_part_add<5>(a, bid, vv, ff);
 }
 
 // ---------------------------------------------------------------------------
@@ -581,6 +665,8 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
 {
     const int bid = blockIdx.x, nb = gridDim.x;
     __shared__ XbSmem sm;
+    extern __shared__ __align__(16) unsigned char xb_dyn_lds[];
+    XbWaveLds *lds = reinterpret_cast<XbWaveLds *>(xb_dyn_lds);
     ph_init(a, bid, nb);
     if (!xb_grid_sync(a.ctl)) return;
     XbLoop st;
@@ -590,20 +676,17 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
         const int32_t L = st.L;
         const unsigned long long t0 = wall_clock64();
         if (st.need_sweep) {
-            ph_compact(a, L, bid, nb, true, &c->sw_n[L & 1], &c->sw_m[L & 1], sm);
+            ph_sweep(a, L, bid, nb, true, true, lds);
             if (!xb_grid_sync(c)) return;
         }
-        long long n_f, m_f;
-        if (st.need_sweep) {
-            n_f = (long long)xb_ld_u64(&c->sw_n[L & 1]);
-            m_f = (long long)xb_ld_u64(&c->sw_m[L & 1]);
-        } else {
-            n_f = (long long)xb_ld_u64(&c->slot_n[L & 3]);
-            m_f = (long long)xb_ld_u64(&c->slot_m[L & 3]);
+        long long sv[5];
+        {
+            const int ff[5] = {st.need_sweep ? xb_f_sweep(L, 0) : xb_f_ring(L, 0),
+                               st.need_sweep ? xb_f_sweep(L, 1) : xb_f_ring(L, 1),
+                               xb_f_ring(L + 1, 0), xb_f_ring(L + 1, 1), xb_f_ring(L + 1, 2)};
+            xb_part_sum<5>(a, ff, sv);
         }
-        const long long pend_n = (long long)xb_ld_u64(&c->slot_n[(L + 1) & 3]);
-        const long long pend_m = (long long)xb_ld_u64(&c->slot_m[(L + 1) & 3]);
-        const long long p2_next = (long long)xb_ld_u64(&c->slot_p2[(L + 1) & 3]);
+        const long long n_f = sv[0], m_f = sv[1], pend_n = sv[2], pend_m = sv[3], p2_next = sv[4];
         long long n_u = 0, m_u = 0;
         const bool open = xb_level_open(st, a, n_f, m_f, pend_n, pend_m, &n_u, &m_u);
         if (!open || L >= a.max_levels) {
@@ -628,12 +711,12 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
                                               a.m, a.alpha, &htag, &used);
         ph_level_prep(a, L, bid, nb);
         if (mode != XBFS_MODE_BU && !st.q_valid && !st.need_sweep) {
-            ph_compact(a, L, bid, nb, false, nullptr, nullptr, sm);
+            ph_sweep(a, L, bid, nb, false, false, lds);
             if (!xb_grid_sync(c)) return;
         }
         if (mode == XBFS_MODE_TD_SF) ph_td<true>(a, L, bid, nb, p2_next == 0, sm);
         else if (mode == XBFS_MODE_TD_SS) ph_td<false>(a, L, bid, nb, false, sm);
-        else ph_bu(a, L, bid, nb, sm);
+        else ph_bu(a, L, bid, nb, lds);
         if (!xb_grid_sync(c)) return;
         if (bid == 0 && threadIdx.x == 0 && L < a.rec_cap) {
             XbDevRec &r = a.recs[L];
@@ -665,29 +748,34 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_coop_level(XbArgs a, int32
                                                                  int32_t enqueue, int32_t extra_sweep)
 {
     __shared__ XbSmem sm;
+    extern __shared__ __align__(16) unsigned char xb_dyn_lds[];
+    XbWaveLds *lds = reinterpret_cast<XbWaveLds *>(xb_dyn_lds);
     const int bid = blockIdx.x, nb = gridDim.x;
     XbCtl *c = a.ctl;
     ph_level_prep(a, L, bid, nb);
     if (mode != XBFS_MODE_BU && !q_valid && !need_sweep) {
-        ph_compact(a, L, bid, nb, false, nullptr, nullptr, sm);
+        ph_sweep(a, L, bid, nb, false, false, lds);
         if (!xb_grid_sync(c)) return;
     }
     if (mode == XBFS_MODE_TD_SF) ph_td<true>(a, L, bid, nb, enqueue != 0, sm);
     else if (mode == XBFS_MODE_TD_SS) ph_td<false>(a, L, bid, nb, false, sm);
-    else ph_bu(a, L, bid, nb, sm);
+    else ph_bu(a, L, bid, nb, lds);
     if (extra_sweep) {
         if (!xb_grid_sync(c)) return;
-        ph_compact(a, L + 1, bid, nb, true, &c->sw_n[(L + 1) & 1], &c->sw_m[(L + 1) & 1], sm);
+        ph_sweep(a, L + 1, bid, nb, true, true, lds);
     }
 }
 
 // ---- per-phase kernels for the host-driven loop ---------------------------
 __global__ void __launch_bounds__(XB_PBLOCK) k_ph_init(XbArgs a) { ph_init(a, blockIdx.x, gridDim.x); }
 
-__global__ void __launch_bounds__(XB_PBLOCK) k_ph_compact(XbArgs a, int32_t L, int32_t stats)
+// from_levels != 0: the sweep after a single-scan level (builds the bits too)
+__global__ void __launch_bounds__(XB_PBLOCK) k_ph_sweep(XbArgs a, int32_t L, int32_t from_levels,
+                                                        int32_t stats)
 {
-    __shared__ XbSmem sm;
-    ph_compact(a, L, blockIdx.x, gridDim.x, stats != 0, &a.ctl->sw_n[L & 1], &a.ctl->sw_m[L & 1], sm);
+    extern __shared__ __align__(16) unsigned char xb_dyn_lds[];
+    ph_sweep(a, L, blockIdx.x, gridDim.x, from_levels != 0, stats != 0,
+             reinterpret_cast<XbWaveLds *>(xb_dyn_lds));
 }
 
 __global__ void __launch_bounds__(XB_PBLOCK) k_ph_prep(XbArgs a, int32_t L)
@@ -709,8 +797,28 @@ __global__ void __launch_bounds__(XB_PBLOCK) k_ph_td_ss(XbArgs a, int32_t L)
 
 __global__ void __launch_bounds__(XB_PBLOCK) k_ph_bu(XbArgs a, int32_t L)
 {
-    __shared__ XbSmem sm;
-    ph_bu(a, L, blockIdx.x, gridDim.x, sm);
+    extern __shared__ __align__(16) unsigned char xb_dyn_lds[];
+    ph_bu(a, L, blockIdx.x, gridDim.x, reinterpret_cast<XbWaveLds *>(xb_dyn_lds));
+}
+
+// Host-driven loop: sums every field of the partial statistics into
+// ctl->sum (one block).
+__global__ void __launch_bounds__(1024) k_part_reduce(XbArgs a)
+{
+    for (int f = 0; f < XB_NFIELDS; f++) {
+        long long acc = 0;
+        for (int b = threadIdx.x; b < XB_PMAX; b += blockDim.x) acc += a.part[(int64_t)f * XB_PMAX + b];
+        __shared__ long long sh[16];
+        const long long t = (long long)xb_wave_sum_u64((unsigned long long)acc);
+        if (xb_lane() == 0) sh[threadIdx.x >> 6] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long u = 0;
+            for (int i = 0; i < (int)(blockDim.x >> 6); i++) u += sh[i];
+            a.ctl->sum[f] = u;
+        }
+        __syncthreads();
+    }
 }
 
 __global__ void k_ctl_reset(XbCtl *c)
