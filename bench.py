@@ -9,7 +9,11 @@ sources with the graph resident in HBM.
 value = harmonic mean over the 64 sources of GTEPS = m_cc / t_bfs / 1e9,
 where m_cc is the sum of degrees of the vertices reached and t_bfs is the
 device time (HIP events) from level initialisation to the final frontier
-check, averaged over the timed steps.
+check, averaged over the timed steps.  Each step runs every source under the
+reference heuristic and, when trained weights are present
+(xbfs_amd/data/policy_mlp.f32), under the learned policy; value is the
+learned policy's figure and "policies" reports both, their ratio, and the
+per-level oracle-best schedule measured on a subset of the sources.
 
 roofline: algorithmic bytes of one traversal, B = 8*m_cc + 20*V_cc + 4*V
 (int32 adjacency word + int32 level probe per traversed edge; int64 offset
@@ -58,13 +62,16 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4,
                     help="sources timed for the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--oracle-best", type=int, default=16,
+                    help="sources on which the per-level oracle-best schedule is measured")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               f"(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
 
-    from xbfs_amd import Context, Graph, default_opts
+    from xbfs_amd import Context, Graph, default_opts, POLICY_HEURISTIC, POLICY_LEARNED
+    from xbfs_amd import policy as P
     t0 = time.time()
     g = Graph.rmat(args.scale, args.ef, 0.57, 0.19, 0.19, seed=1000 + args.scale)
     t_gen = time.time() - t0
@@ -74,36 +81,62 @@ def main():
     kw = {}
     if args.persistent >= 0:
         kw["persistent"] = args.persistent
-    opts = default_opts(**kw)
+    have_model = os.path.exists(P.WEIGHTS)
+    if have_model:
+        ctx.load_policy(P.load())
+    pols = [("heuristic", default_opts(policy=POLICY_HEURISTIC, **kw))]
+    if have_model:
+        pols.append(("learned", default_opts(policy=POLICY_LEARNED, **kw)))
+    opts = pols[-1][1]
     print(f"[bench] RMAT-{args.scale} ef{args.ef}: n={g.n} m={g.m} gen {t_gen:.1f}s; "
-          f"{len(srcs)} sources", file=sys.stderr, flush=True)
+          f"{len(srcs)} sources; policies {[p for p, _ in pols]}", file=sys.stderr, flush=True)
 
     def one_step():
-        ts, st_all = [], []
+        ts = {p: [] for p, _ in pols}
+        st_all = []
         for s in srcs:
-            _, _, _, st = ctx.bfs(int(s), opts=opts, want_levels=False)
-            ts.append(st["t_bfs_s"])
+            for p, o in pols:
+                _, _, _, st = ctx.bfs(int(s), opts=o, want_levels=False)
+                ts[p].append(st["t_bfs_s"])
             st_all.append(st)
         return ts, st_all
 
     for _ in range(args.warmup):
         one_step()
-    t_dev = np.zeros(len(srcs))
+    t_dev = {p: np.zeros(len(srcs)) for p, _ in pols}
     st_ref = None
     w0 = time.perf_counter()
     for k in range(args.steps):
         ts, sts = one_step()
-        t_dev += np.asarray(ts)
+        for p in ts:
+            t_dev[p] += np.asarray(ts[p])
         st_ref = sts
         print(f"[bench] step {k} done", file=sys.stderr, flush=True)
     wall = time.perf_counter() - w0
-    t_dev /= max(args.steps, 1)
+    for p in t_dev:
+        t_dev[p] /= max(args.steps, 1)
     m_cc = np.array([s["m_cc"] for s in st_ref], np.float64)
     v_cc = np.array([s["v_cc"] for s in st_ref], np.float64)
-    gteps = m_cc / t_dev / 1e9
+    main_pol = pols[-1][0]
+    gteps = m_cc / t_dev[main_pol] / 1e9
     value = hmean(gteps)
+    value_heur = hmean(m_cc / t_dev["heuristic"] / 1e9)
     b_alg = 8 * m_cc + 20 * v_cc + 4 * g.n
-    achieved = float(b_alg.sum() / t_dev.sum())
+    achieved = float(b_alg.sum() / t_dev[main_pol].sum())
+    policy_cmp = {"heuristic_gteps_hmean": round(value_heur, 3),
+                  "heuristic_total_ms": round(float(t_dev["heuristic"].sum()) * 1e3, 3)}
+    if have_model:
+        policy_cmp["learned_gteps_hmean"] = round(value, 3)
+        policy_cmp["learned_total_ms"] = round(float(t_dev["learned"].sum()) * 1e3, 3)
+        policy_cmp["learned_speedup_vs_heuristic"] = round(
+            float(t_dev["heuristic"].sum() / t_dev["learned"].sum()), 4)
+        if args.oracle_best > 0:
+            sub = srcs[: args.oracle_best]
+            tot = P.evaluate(ctx, sub, P.load(), reps=3)
+            policy_cmp["oracle_best_sample"] = {
+                "sources": int(len(sub)), "heuristic_ms": round(tot["heuristic"] * 1e3, 3),
+                "learned_ms": round(tot["learned"] * 1e3, 3),
+                "oracle_best_ms": round(tot["oracle_best"] * 1e3, 3)}
 
     # correctness spot check of one source against the oracle validator
     from oracle import oracle as O
@@ -136,12 +169,16 @@ def main():
         "data": "synthetic (seeded RMAT generated in-process)",
         "config": {"workload": f"RMAT scale {args.scale} ef{args.ef} A/B/C/D=.57/.19/.19/.05, "
                                f"{len(srcs)} seeded sources, single-source BFS each",
-                   "n": g.n, "m": g.m, "policy": "heuristic (reference alpha rule, alpha=0.1)",
-                   "persistent": int(opts.persistent), "levels_valid": bool(valid)},
+                   "n": g.n, "m": g.m, "persistent": int(opts.persistent),
+                   "policy": ("learned MLP cost model (heuristic fallback)" if have_model
+                              else "heuristic (reference alpha rule, alpha=0.1)"),
+                   "levels_valid": bool(valid)},
+        "policies": policy_cmp,
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
                      "kernel": "k_persistent" if opts.persistent else "host-driven level loop",
-                     "launches_per_bfs": 1 if opts.persistent else None},
+                     "launches_per_bfs": 1 if opts.persistent else None,
+                     "avg_launch_us": round(float(t_dev[main_pol].mean()) * 1e6, 2)},
         "cpu_baseline": cpu,
         "gteps_min": round(float(gteps.min()), 3),
         "gteps_max": round(float(gteps.max()), 3),
