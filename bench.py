@@ -37,6 +37,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
 
+# HBM bytes per k_persistent launch on the default workload (RMAT-24 ef16, the
+# 64 seeded sources, learned policy), from two rocprofv3 --pmc passes
+# (profiles/k_persistent_rmat24_pmc_traffic.md): mean FETCH_SIZE and
+# WRITE_SIZE in KiB per dispatch, each scaled by what the same counter reads
+# for a plain 1 GiB device copy (FETCH_SIZE reads half of the bytes a copy
+# moves: factor 2.0; WRITE_SIZE reads all of them: factor 1.0).
+PMC_TRAFFIC = {"workload": (24, 16, 64, "learned"), "fetch_kib": 846238.7, "fetch_factor": 1.99995,
+               "write_kib": 265817.7, "write_factor": 1.0}
+
 
 def hmean(x):
     x = np.asarray(x, np.float64)
@@ -64,11 +73,27 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--oracle-best", type=int, default=16,
                     help="sources on which the per-level oracle-best schedule is measured")
+    ap.add_argument("--compare", type=int, default=1,
+                    help="1 = also time the heuristic policy; 0 = only the main policy "
+                         "(profiling runs, so every k_persistent launch is the measured one)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="run only a plain device-to-device copy of BYTES bytes, 5 times "
+                         "(scale factor for the FETCH_SIZE / WRITE_SIZE counters), and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              f"(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+              f"(sharding a traversal over several GPUs is not built)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy > 0:
+        import torch
+        a = torch.empty(args.calibrate_copy // 4, dtype=torch.int32, device="cuda:0").fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": a.numel() * 4, "launches": 5}))
+        return
 
     from xbfs_amd import Context, Graph, default_opts, POLICY_HEURISTIC, POLICY_LEARNED
     from xbfs_amd import policy as P
@@ -87,6 +112,8 @@ def main():
     pols = [("heuristic", default_opts(policy=POLICY_HEURISTIC, **kw))]
     if have_model:
         pols.append(("learned", default_opts(policy=POLICY_LEARNED, **kw)))
+        if not args.compare:
+            pols = pols[1:]
     opts = pols[-1][1]
     print(f"[bench] RMAT-{args.scale} ef{args.ef}: n={g.n} m={g.m} gen {t_gen:.1f}s; "
           f"{len(srcs)} sources; policies {[p for p, _ in pols]}", file=sys.stderr, flush=True)
@@ -120,12 +147,17 @@ def main():
     main_pol = pols[-1][0]
     gteps = m_cc / t_dev[main_pol] / 1e9
     value = hmean(gteps)
-    value_heur = hmean(m_cc / t_dev["heuristic"] / 1e9)
     b_alg = 8 * m_cc + 20 * v_cc + 4 * g.n
     achieved = float(b_alg.sum() / t_dev[main_pol].sum())
-    policy_cmp = {"heuristic_gteps_hmean": round(value_heur, 3),
-                  "heuristic_total_ms": round(float(t_dev["heuristic"].sum()) * 1e3, 3)}
-    if have_model:
+    traffic = None
+    if PMC_TRAFFIC["workload"] == (args.scale, args.ef, len(srcs), main_pol):
+        traffic = 1024.0 * (PMC_TRAFFIC["fetch_kib"] * PMC_TRAFFIC["fetch_factor"] +
+                            PMC_TRAFFIC["write_kib"] * PMC_TRAFFIC["write_factor"])
+    policy_cmp = {}
+    if "heuristic" in t_dev:
+        policy_cmp["heuristic_gteps_hmean"] = round(hmean(m_cc / t_dev["heuristic"] / 1e9), 3)
+        policy_cmp["heuristic_total_ms"] = round(float(t_dev["heuristic"].sum()) * 1e3, 3)
+    if have_model and "heuristic" in t_dev:
         policy_cmp["learned_gteps_hmean"] = round(value, 3)
         policy_cmp["learned_total_ms"] = round(float(t_dev["learned"].sum()) * 1e3, 3)
         policy_cmp["learned_speedup_vs_heuristic"] = round(
@@ -175,7 +207,11 @@ def main():
                    "levels_valid": bool(valid)},
         "policies": policy_cmp,
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": None if traffic is None else round(traffic),
+                     "algorithmic_bytes_per_launch": round(float(b_alg.mean())),
+                     "measured_hbm_gbs": (None if traffic is None else
+                                          round(traffic / float(t_dev[main_pol].mean()) / 1e9, 1)),
                      "kernel": "k_persistent" if opts.persistent else "host-driven level loop",
                      "launches_per_bfs": 1 if opts.persistent else None,
                      "avg_launch_us": round(float(t_dev[main_pol].mean()) * 1e6, 2)},
