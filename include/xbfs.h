@@ -35,10 +35,11 @@
  *   xbfs_unvisited_list
  *       ballot_counting + FQ_gen, bfs_adaptiveFQ_wb_async.cuh:189-284: the
  *       ascending list of unvisited vertices, built by 64-bit wave ballots.
- *   xbfs_bfs_batch64, xbfs_bfs_multigraph, xbfs_policy_*, xbfs_mlp_*
- *       no reference counterpart (batched traversal and the learned
- *       direction/queue policy that replaces the decision block
- *       bfs_adaptiveFQ_wb_async.cuh:1047-1078).
+ *   xbfs_bfs_multi, xbfs_bfs_multigraph, xbfs_graph_concat,
+ *   xbfs_bfs_batch64, xbfs_autotune, xbfs_policy_*, xbfs_mlp_*
+ *       no reference counterpart (batched traversal, the bin/grid
+ *       autotuner, and the learned direction/queue policy that replaces the
+ *       decision block bfs_adaptiveFQ_wb_async.cuh:1047-1078).
  */
 #ifndef XBFS_H
 #define XBFS_H
@@ -181,6 +182,55 @@ typedef struct {
 int xbfs_bfs(xbfs_ctx *ctx, int32_t src, const xbfs_opts *opts,
              int32_t *level_out, int32_t *parent_out,
              xbfs_level_rec *rec, int32_t rec_cap, xbfs_stats *st);
+
+/* ---- batched traversal (no reference counterpart) ------------------------ */
+
+/* Multi-source BFS: level[v] = hops from v to the nearest of the nsrc seeds
+ * (duplicates allowed), -1 when no seed reaches v; parents form a BFS forest
+ * rooted at the seeds (parent[seed] = seed).  Same level loop, policies,
+ * records and stats as xbfs_bfs (v_cc counts every distinct seed). */
+int xbfs_bfs_multi(xbfs_ctx *ctx, const int32_t *srcs, int32_t nsrc, const xbfs_opts *opts,
+                   int32_t *level_out, int32_t *parent_out,
+                   xbfs_level_rec *rec, int32_t rec_cap, xbfs_stats *st);
+
+/* Packs k host graphs side by side into one CSR: vertex v of gs[i] becomes
+ * vert_off[i] + v (vert_off: k+1 entries, vert_off[k] = total vertices). */
+int xbfs_graph_concat(const xbfs_host_graph *gs, int32_t k, xbfs_host_graph *out,
+                      int64_t *vert_off);
+
+/* Multigraph batch: ctx holds k graphs packed by xbfs_graph_concat; one BFS
+ * per graph from its local source local_src[i], all k in one traversal (one
+ * persistent launch).  level_out: host int32[total n]; the slice
+ * [vert_off[i], vert_off[i+1]) is graph i's level array in its own vertex
+ * numbering.  Components are disjoint, so every slice equals graph i's
+ * single-source BFS. */
+int xbfs_bfs_multigraph(xbfs_ctx *ctx, const int64_t *vert_off, int32_t k,
+                        const int32_t *local_src, const xbfs_opts *opts,
+                        int32_t *level_out, xbfs_stats *st);
+
+/* 64-source batch: up to 64 single-source BFSs at once, source s as bit s of
+ * a 64-bit frontier / visited mask per vertex, expanded top-down (push) or
+ * bottom-up (pull) per level by the union frontier's size.
+ * levels_out: host int32[nsrc * n], row s = levels from srcs[s], or NULL
+ * (rows stay on the device).  st (may be NULL): t_bfs_s = device time of the
+ * whole batch, v_cc / m_cc = sums over the sources, n_levels = levels run. */
+int xbfs_bfs_batch64(xbfs_ctx *ctx, const int32_t *srcs, int32_t nsrc, const xbfs_opts *opts,
+                     int32_t *levels_out, xbfs_stats *st);
+
+/* Autotuner for the degree-bin thresholds and the grid size: coordinate
+ * descent (two passes) over candidate values of bin_thread_max,
+ * bin_wave_max, bin_block_max and grid_dim.  Objective: multi = 0, the sum
+ * over the nsrc sample sources of single-source t_bfs; multi = 1, the t_bfs
+ * of one multi-source traversal from all of them (the multigraph batch).
+ * Each measurement is the median of reps runs.  opts is the starting point
+ * on entry and the best setting found on return.  trials (may be NULL)
+ * receives up to cap records; *ntrials (may be NULL) = records written. */
+typedef struct {
+    int32_t bin_thread_max, bin_wave_max, bin_block_max, grid_dim;
+    double t_s;          /* summed traversal time of the sample sources */
+} xbfs_tune_rec;
+int xbfs_autotune(xbfs_ctx *ctx, const int32_t *srcs, int32_t nsrc, int32_t multi, int32_t reps,
+                  xbfs_opts *opts, xbfs_tune_rec *trials, int32_t cap, int32_t *ntrials);
 
 /* Device scan used by the frontier generators, exposed for testing:
  * out[i] = sum(in[0..i-1]) (int64), n up to 2^31-1.  Host pointers. */

@@ -31,6 +31,13 @@
  *                        int32 levels: the CPU baseline.  Pinned by
  *                        agreement with oracle_bfs_queue.
  *   oracle_exclusive_scan  std::exclusive_scan restated (int32 in, int64 out).
+ *   oracle_bfs_multi     serial FIFO BFS from a set of seeds (the batched
+ *                        modes' oracle; no reference counterpart).  Pinned by
+ *                        the identity level = min over seeds of the
+ *                        single-source oracle level, and a closed form (path
+ *                        with two seeds: min(|i-a|, |i-b|)).
+ *   oracle_validate_multi  oracle_validate with a seed set (level 0 exactly
+ *                        at the seeds); pinned by planted mistakes.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -95,6 +102,93 @@ int32_t oracle_bfs_queue(int64_t n, const int64_t *beg, const int32_t *adj,
     }
     free(q);
     return maxl + 1;
+}
+
+/* Serial FIFO BFS from every distinct seed at once (level 0, parent = the
+ * seed itself); level[v] = hops to the nearest seed.  Returns max level + 1
+ * (0 when k == 0). */
+int32_t oracle_bfs_multi(int64_t n, const int64_t *beg, const int32_t *adj,
+                         const int32_t *srcs, int32_t k, int32_t *level, int32_t *parent)
+{
+    int32_t *q = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++) level[i] = -1;
+    if (parent)
+        for (int64_t i = 0; i < n; i++) parent[i] = -1;
+    int64_t head = 0, tail = 0;
+    for (int32_t i = 0; i < k; i++) {
+        int32_t s = srcs[i];
+        if (level[s] == -1) {
+            level[s] = 0;
+            if (parent) parent[s] = s;
+            q[tail++] = s;
+        }
+    }
+    int32_t maxl = k > 0 ? 0 : -1;
+    while (head < tail) {
+        int32_t u = q[head++];
+        int32_t lu = level[u];
+        if (lu > maxl) maxl = lu;
+        for (int64_t j = beg[u]; j < beg[u + 1]; j++) {
+            int32_t v = adj[j];
+            if (level[v] == -1) {
+                level[v] = lu + 1;
+                if (parent) parent[v] = u;
+                q[tail++] = v;
+            }
+        }
+    }
+    free(q);
+    return maxl + 1;
+}
+
+/* Validation of a multi-source result; returns the number of violated checks.
+ *  (1) level[v] == 0 exactly for the seeds (parent[seed] == seed);
+ *  (2) every edge (u,v) with u reached has v reached, |level[u]-level[v]| <= 1;
+ *  (3) every reached non-seed v has a neighbour at level[v]-1 (and, with
+ *      parents, parent[v] is such a neighbour); unreached v has parent -1.
+ * Together these make level the distance to the nearest seed. */
+int64_t oracle_validate_multi(int64_t n, const int64_t *beg, const int32_t *adj,
+                              const int32_t *srcs, int32_t k, const int32_t *level,
+                              const int32_t *parent)
+{
+    int64_t bad = 0;
+    unsigned char *seed = (unsigned char *)calloc((size_t)(n > 0 ? n : 1), 1);
+    for (int32_t i = 0; i < k; i++) seed[srcs[i]] = 1;
+    #pragma omp parallel for reduction(+:bad) schedule(dynamic, 4096)
+    for (int64_t u = 0; u < n; u++) {
+        int32_t lu = level[u];
+        if (seed[u]) {
+            if (lu != 0) bad++;
+            if (parent && parent[u] != u) bad++;
+        } else if (lu == 0) {
+            bad++;
+        }
+        if (lu < -1) { bad++; continue; }
+        if (lu == -1) {
+            if (parent && parent[u] != -1) bad++;
+            continue;
+        }
+        int has_pred = seed[u];
+        for (int64_t j = beg[u]; j < beg[u + 1]; j++) {
+            int32_t v = adj[j];
+            int32_t lv = level[v];
+            if (lv == -1) { bad++; continue; }
+            int32_t d = lu - lv;
+            if (d > 1 || d < -1) bad++;
+            if (lv == lu - 1) has_pred = 1;
+        }
+        if (!has_pred) bad++;
+        if (parent && !seed[u]) {
+            int32_t p = parent[u];
+            if (p < 0 || p >= n || level[p] != lu - 1) { bad++; continue; }
+            int found = 0;
+            for (int64_t j = beg[u]; j < beg[u + 1]; j++)
+                if (adj[j] == p) { found = 1; break; }
+            if (!found) bad++;
+        }
+    }
+    free(seed);
+    return bad;
 }
 
 /* Graph500-style validation of a BFS result.  Returns the number of violated

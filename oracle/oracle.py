@@ -43,6 +43,12 @@ def lib():
         L.oracle_omp_td_bfs.argtypes = [c_int64, c_void_p, c_void_p, c_int32, c_void_p, c_int32]
         L.oracle_exclusive_scan.restype = None
         L.oracle_exclusive_scan.argtypes = [c_int64, c_void_p, c_void_p]
+        L.oracle_bfs_multi.restype = c_int32
+        L.oracle_bfs_multi.argtypes = [c_int64, c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
+                                       c_void_p]
+        L.oracle_validate_multi.restype = c_int64
+        L.oracle_validate_multi.argtypes = [c_int64, c_void_p, c_void_p, c_void_p, c_int32,
+                                            c_void_p, c_void_p]
         L.oracle_max_threads.restype = c_int32
         L.oracle_mlp_nparam.restype = c_int32
         L.oracle_mlp_forward.restype = None
@@ -82,6 +88,28 @@ def bfs_queue(beg, adj, src, parents=False):
     par = np.empty(n, np.int32) if parents else None
     k = lib().oracle_bfs_queue(n, _p(beg), _p(adj), int(src), _p(lv), _p(par))
     return lv, par, k
+
+
+def bfs_multi(beg, adj, srcs, parents=False):
+    """Serial FIFO BFS from every seed at once.  Returns (levels, parents or None, nlevels)."""
+    beg, adj = _csr(beg, adj)
+    n = beg.shape[0] - 1
+    s = np.ascontiguousarray(srcs, np.int32)
+    lv = np.empty(n, np.int32)
+    par = np.empty(n, np.int32) if parents else None
+    k = lib().oracle_bfs_multi(n, _p(beg), _p(adj), _p(s), s.shape[0], _p(lv), _p(par))
+    return lv, par, k
+
+
+def validate_multi(beg, adj, srcs, level, parent=None):
+    """Number of violated checks of a multi-source result (0 = valid)."""
+    beg, adj = _csr(beg, adj)
+    n = beg.shape[0] - 1
+    s = np.ascontiguousarray(srcs, np.int32)
+    level = np.ascontiguousarray(level, np.int32)
+    parent = None if parent is None else np.ascontiguousarray(parent, np.int32)
+    return int(lib().oracle_validate_multi(n, _p(beg), _p(adj), _p(s), s.shape[0], _p(level),
+                                           _p(parent)))
 
 
 def validate(beg, adj, src, level, parent=None):

@@ -152,3 +152,31 @@ def test_header_symbols_exported():
     for nm in sorted(names):
         assert hasattr(L, nm), f"{nm} not exported"
         assert nm in _capi.SIGNATURES, f"{nm} has no binding"
+
+
+def test_concat_is_disjoint_union():
+    """xbfs_graph_concat: graph i's vertex v is vert_off[i] + v; each slice
+    of a multi-source BFS over the union from one seed per graph equals the
+    graph's own single-source BFS."""
+    gs = [Graph.grid(3, 5), Graph.rmat(7, 4, seed=11), Graph(np.zeros(2, np.int64),
+                                                             np.zeros(0, np.int32)),
+          Graph.rmat(6, 8, seed=12)]
+    u, off = Graph.concat(gs)
+    assert off.tolist() == np.cumsum([0] + [g.n for g in gs]).tolist()
+    assert u.n == sum(g.n for g in gs) and u.m == sum(g.m for g in gs)
+    csr_invariants(u)
+    want_beg, want_adj = G.disjoint(*[(g.beg_pos, g.adj) for g in gs])
+    assert (u.beg_pos == want_beg).all() and (u.adj == want_adj).all()
+    srcs = [0, int(np.argmax(gs[1].degrees())), 0, 5]
+    lv, _, _ = O.bfs_multi(u.beg_pos, u.adj, [off[i] + s for i, s in enumerate(srcs)])
+    for i, g in enumerate(gs):
+        want, _, _ = O.bfs_queue(g.beg_pos, g.adj, srcs[i])
+        assert (lv[off[i]:off[i + 1]] == want).all()
+
+
+def test_concat_rejects_bad_input():
+    bad = Graph(np.array([0, 2], np.int64), np.zeros(1, np.int32))  # beg[n] != m
+    with pytest.raises(XbfsError):
+        Graph.concat([Graph.grid(2, 2), bad])
+    with pytest.raises(XbfsError):
+        Graph.concat([])

@@ -199,3 +199,60 @@ def test_mlp_grad_finite_difference():
     P2[-1] = 0.5                      # b3[2]
     y = O.mlp_forward(P2, np.array([[1.5] + [0] * 7, [-1.0] + [0] * 7], np.float32))
     assert np.allclose(y, [[9.0, 0, 0.5], [0, 0, 0.5]])
+
+
+# ---- multi-source oracle (batched modes) ---------------------------------------
+@pytest.mark.parametrize("seed", range(6))
+def test_multi_source_is_min_of_single_sources(seed):
+    """Distance to the nearest seed = min over seeds of the single-source
+    distance (brute force over the seeds; -1 = unreached)."""
+    rng = np.random.default_rng(seed)
+    beg, adj = G.disjoint(G.random_graph(40, 0.06, seed), G.random_graph(25, 0.1, seed + 50),
+                          G.path(7))
+    n = beg.shape[0] - 1
+    k = int(rng.integers(1, 9))
+    srcs = rng.integers(0, n, size=k).astype(np.int32)
+    srcs = np.concatenate([srcs, srcs[:1]])  # a duplicate
+    lv, par, _ = O.bfs_multi(beg, adj, srcs, parents=True)
+    singles = np.stack([O.bfs_queue(beg, adj, int(s))[0] for s in srcs])
+    big = np.where(singles < 0, 10 ** 9, singles).min(axis=0)
+    want = np.where(big >= 10 ** 9, -1, big)
+    assert (lv == want).all()
+    assert O.validate_multi(beg, adj, srcs, lv, par) == 0
+
+
+def test_multi_source_path_closed_form():
+    beg, adj = G.path(50)
+    for a, b in ((0, 49), (10, 11), (25, 25), (3, 40)):
+        lv, _, k = O.bfs_multi(beg, adj, [a, b])
+        i = np.arange(50)
+        assert (lv == np.minimum(np.abs(i - a), np.abs(i - b))).all()
+        assert k == lv.max() + 1
+
+
+def test_multi_source_isolated_seed():
+    beg, adj = G.disjoint(G.path(5), (np.zeros(2, np.int64), np.zeros(0, np.int32)))
+    lv, par, _ = O.bfs_multi(beg, adj, [5, 2], parents=True)
+    assert lv.tolist() == [2, 1, 0, 1, 2, 0]
+    assert par[5] == 5 and par[2] == 2
+    assert O.validate_multi(beg, adj, [5, 2], lv, par) == 0
+
+
+def test_validate_multi_catches_planted_mistakes():
+    beg, adj = G.grid(6, 7)
+    srcs = np.array([0, 41], np.int32)
+    lv, par, _ = O.bfs_multi(beg, adj, srcs, parents=True)
+    assert O.validate_multi(beg, adj, srcs, lv, par) == 0
+    bad = lv.copy(); bad[20] += 1
+    assert O.validate_multi(beg, adj, srcs, bad) > 0          # wrong level
+    bad = lv.copy(); bad[41] = 1
+    assert O.validate_multi(beg, adj, srcs, bad) > 0          # seed not at 0
+    bad = lv.copy(); bad[20] = 0
+    assert O.validate_multi(beg, adj, srcs, bad) > 0          # non-seed at 0
+    bad = lv.copy(); bad[7] = -1
+    assert O.validate_multi(beg, adj, srcs, bad) > 0          # missing vertex
+    assert O.validate_multi(beg, adj, srcs[:1], lv) > 0       # levels of another seed set
+    p = par.copy(); p[20] = 20
+    assert O.validate_multi(beg, adj, srcs, lv, p) > 0        # self parent
+    p = par.copy(); p[20] = 0
+    assert O.validate_multi(beg, adj, srcs, lv, p) > 0        # non-adjacent parent

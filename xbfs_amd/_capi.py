@@ -62,6 +62,11 @@ class Stats(Structure):
                 ("gen_s", c_double)]
 
 
+class TuneRec(Structure):
+    _fields_ = [("bin_thread_max", c_int32), ("bin_wave_max", c_int32),
+                ("bin_block_max", c_int32), ("grid_dim", c_int32), ("t_s", c_double)]
+
+
 class ProbeRec(Structure):
     _fields_ = [("level", c_int32), ("prev_mode", c_int32), ("mode", c_int32), ("htag", c_int32),
                 ("n_f", c_int64), ("m_f", c_int64), ("n_u", c_int64), ("m_u", c_int64),
@@ -101,6 +106,15 @@ SIGNATURES = {
                                c_float, c_void_p]),
     "xbfs_bfs_probe": (c_int, [c_void_p, c_int32, POINTER(Opts), c_int32, c_int32,
                                POINTER(ProbeRec), c_int32, POINTER(Stats)]),
+    "xbfs_bfs_multi": (c_int, [c_void_p, c_void_p, c_int32, POINTER(Opts), c_void_p, c_void_p,
+                               POINTER(LevelRec), c_int32, POINTER(Stats)]),
+    "xbfs_graph_concat": (c_int, [POINTER(HostGraph), c_int32, POINTER(HostGraph), c_void_p]),
+    "xbfs_bfs_multigraph": (c_int, [c_void_p, c_void_p, c_int32, c_void_p, POINTER(Opts),
+                                    c_void_p, POINTER(Stats)]),
+    "xbfs_bfs_batch64": (c_int, [c_void_p, c_void_p, c_int32, POINTER(Opts), c_void_p,
+                                 POINTER(Stats)]),
+    "xbfs_autotune": (c_int, [c_void_p, c_void_p, c_int32, c_int32, c_int32, POINTER(Opts),
+                              POINTER(TuneRec), c_int32, POINTER(c_int32)]),
 }
 
 _lib = None
@@ -203,6 +217,19 @@ class Graph:
     def degrees(self):
         return np.diff(self.beg_pos)
 
+    @staticmethod
+    def concat(graphs):
+        """Disjoint union (xbfs_graph_concat).  Returns (graph, vert_off) where
+        graph i's vertex v is vert_off[i] + v."""
+        k = len(graphs)
+        hosts = (HostGraph * k)()
+        for i, g in enumerate(graphs):
+            hosts[i] = g._host()
+        off = np.empty(k + 1, np.int64)
+        out = HostGraph()
+        check(lib().xbfs_graph_concat(hosts, k, byref(out), _ptr(off)), "graph_concat")
+        return Graph._from_host(out), off
+
 
 def default_opts(**kw):
     o = Opts()
@@ -259,6 +286,63 @@ class Context:
                 out_recs.append({k: getattr(r, k) for k, _ in LevelRec._fields_})
         stats = {k: getattr(st, k) for k, _ in Stats._fields_}
         return lv, par, out_recs, stats
+
+    def bfs_multi(self, srcs, opts=None, want_levels=True, want_parent=False, rec_cap=0, **kw):
+        """Multi-source BFS (distance to the nearest seed).  Returns
+        (levels, parents, records, stats)."""
+        o = opts if opts is not None else default_opts(**kw)
+        s = np.ascontiguousarray(srcs, dtype=np.int32)
+        lv = np.empty(self.n, np.int32) if want_levels else None
+        par = np.empty(self.n, np.int32) if want_parent else None
+        recs = (LevelRec * rec_cap)() if rec_cap > 0 else None
+        st = Stats()
+        check(lib().xbfs_bfs_multi(self.h, _ptr(s), s.shape[0], byref(o),
+                                   _ptr(lv) if lv is not None else None,
+                                   _ptr(par) if par is not None else None, recs, rec_cap,
+                                   byref(st)), "xbfs_bfs_multi")
+        out_recs = []
+        if recs is not None:
+            for i in range(min(rec_cap, st.n_levels)):
+                r = recs[i]
+                out_recs.append({k: getattr(r, k) for k, _ in LevelRec._fields_})
+        return lv, par, out_recs, {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def bfs_multigraph(self, vert_off, local_src, opts=None, want_levels=True, **kw):
+        """One BFS per packed graph (xbfs_bfs_multigraph).  Returns (levels, stats);
+        levels[vert_off[i]:vert_off[i+1]] is graph i's level array."""
+        o = opts if opts is not None else default_opts(**kw)
+        off = np.ascontiguousarray(vert_off, dtype=np.int64)
+        ls = np.ascontiguousarray(local_src, dtype=np.int32)
+        lv = np.empty(self.n, np.int32) if want_levels else None
+        st = Stats()
+        check(lib().xbfs_bfs_multigraph(self.h, _ptr(off), ls.shape[0], _ptr(ls), byref(o),
+                                        _ptr(lv) if lv is not None else None, byref(st)),
+              "xbfs_bfs_multigraph")
+        return lv, {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def bfs_batch64(self, srcs, opts=None, want_levels=True, **kw):
+        """Up to 64 single-source BFSs in one bit-parallel traversal.  Returns
+        (levels [nsrc, n] or None, stats)."""
+        o = opts if opts is not None else default_opts(**kw)
+        s = np.ascontiguousarray(srcs, dtype=np.int32)
+        lv = np.empty((s.shape[0], self.n), np.int32) if want_levels else None
+        st = Stats()
+        check(lib().xbfs_bfs_batch64(self.h, _ptr(s), s.shape[0], byref(o),
+                                     _ptr(lv) if lv is not None else None, byref(st)),
+              "xbfs_bfs_batch64")
+        return lv, {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def autotune(self, srcs, multi=False, reps=3, opts=None, cap=256, **kw):
+        """Coordinate-descent search of bin thresholds and grid size
+        (xbfs_autotune).  Returns (best opts, list of trial dicts)."""
+        o = opts if opts is not None else default_opts(**kw)
+        s = np.ascontiguousarray(srcs, dtype=np.int32)
+        trials = (TuneRec * cap)()
+        nt = c_int32()
+        check(lib().xbfs_autotune(self.h, _ptr(s), s.shape[0], int(bool(multi)), int(reps),
+                                  byref(o), trials, cap, byref(nt)), "xbfs_autotune")
+        out = [{k: getattr(trials[i], k) for k, _ in TuneRec._fields_} for i in range(nt.value)]
+        return o, out
 
     def load_policy(self, weights):
         w = np.ascontiguousarray(weights, dtype=np.float32)

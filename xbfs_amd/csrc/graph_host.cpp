@@ -521,3 +521,49 @@ int xbfs_check_csr(int64_t n, int64_t m, const int64_t *beg, const int32_t *adj)
 }
 
 } // extern "C"
+
+// Disjoint union of k graphs: graph i's vertex v becomes vert_off[i] + v and
+// its CSR rows follow graph i-1's.  Weights are dropped (the traversal does
+// not read them).
+extern "C" int xbfs_graph_concat(const xbfs_host_graph *gs, int32_t k, xbfs_host_graph *out,
+                                 int64_t *vert_off)
+{
+    if (!gs || k <= 0 || !out || !vert_off) return XBFS_ERR_ARG;
+    int64_t n = 0, m = 0;
+    for (int32_t i = 0; i < k; i++) {
+        if (gs[i].n < 0 || gs[i].m < 0 || (gs[i].n > 0 && !gs[i].beg_pos) ||
+            (gs[i].m > 0 && !gs[i].adj))
+            return XBFS_ERR_ARG;
+        if (gs[i].beg_pos && (gs[i].beg_pos[0] != 0 || gs[i].beg_pos[gs[i].n] != gs[i].m))
+            return XBFS_ERR_FORMAT;
+        vert_off[i] = n;
+        n += gs[i].n;
+        m += gs[i].m;
+    }
+    vert_off[k] = n;
+    if (n > INT32_MAX) return XBFS_ERR_LIMIT;
+    int64_t *beg = alloc_array<int64_t>(n + 1);
+    int32_t *adj = alloc_array<int32_t>(m > 0 ? m : 1);
+    if (!beg || !adj) {
+        free(beg);
+        free(adj);
+        return XBFS_ERR_NOMEM;
+    }
+    std::vector<int64_t> eoff((size_t)k + 1, 0);
+    for (int32_t i = 0; i < k; i++) eoff[(size_t)i + 1] = eoff[(size_t)i] + gs[i].m;
+    beg[0] = 0;
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int32_t i = 0; i < k; i++) {
+        const xbfs_host_graph &g = gs[i];
+        const int64_t v0 = vert_off[i], e0 = eoff[(size_t)i];
+        const int32_t shift = (int32_t)v0;
+        for (int64_t v = 0; v < g.n; v++) beg[v0 + v + 1] = e0 + g.beg_pos[v + 1];
+        for (int64_t j = 0; j < g.m; j++) adj[e0 + j] = g.adj[j] + shift;
+    }
+    out->n = n;
+    out->m = m;
+    out->beg_pos = beg;
+    out->adj = adj;
+    out->weight = nullptr;
+    return XBFS_OK;
+}

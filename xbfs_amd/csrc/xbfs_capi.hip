@@ -16,6 +16,7 @@
 #include "xbfs_kernels.hip"
 #include "xbfs_engine.hip"
 #include "xbfs_mlp.hip"
+#include "xbfs_batch.hip"
 
 // dynamic LDS: one XbWaveLds per wave of the block
 #define XB_LDS_PHASE (sizeof(XbWaveLds) * (XB_PBLOCK / 64))
@@ -50,13 +51,17 @@ int xb_ctx_free_device(xbfs_ctx *c)
     void *bufs[] = {c->d_beg,   c->d_adj,  c->d_nz,    c->d_vis,  c->d_fbm,   c->d_level,
                     c->d_parent, c->d_qbuf, c->d_ctl,  c->d_recs, c->d_sched, c->d_mlp,
                     c->d_list,  c->d_ubits, c->d_tile, c->d_list_cnt, c->s_level,
-                    c->s_vis,   c->s_fbm,   c->s_qbuf, c->s_ctl, c->d_part, c->s_part};
+                    c->s_vis,   c->s_fbm,   c->s_qbuf, c->s_ctl, c->d_part, c->s_part,
+                    c->d_srcs,  c->d_b64_deg, c->d_b64_segu, c->d_b64_sege, c->d_b64_F,
+                    c->d_b64_acc, c->d_b64_hist, c->d_b64_lv, c->d_b64_out, c->d_b64_srcs};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->ev_start) (void)hipEventDestroy(c->ev_start);
     if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
+    for (hipEvent_t e : c->ev_b64)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     return XBFS_OK;
 }
@@ -233,6 +238,8 @@ static XbArgs make_args(xbfs_ctx *c, int32_t src, const xbfs_opts &o)
     a.ntiles = c->ntiles_bu;
     a.fstride = c->fstride;
     a.n_nz = c->n_nz;
+    a.srcs = nullptr;
+    a.nsrc = 0;
     a.src = src;
     a.t1 = o.bin_thread_max;
     a.t2 = o.bin_wave_max;
@@ -290,9 +297,11 @@ static int bfs_host_loop(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs_
     if (ensure_events(c, 4) != XBFS_OK) return XBFS_ERR_HIP;
     XB_HIP(hipEventRecord(c->ev_start, c->stream));
     hipLaunchKernelGGL(k_ph_init, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a);
+    if (a.nsrc > 0) hipLaunchKernelGGL(k_ph_seed, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a);
     XB_LAUNCHED();
     XbLoop s;
     xb_loop_init(&s);
+    if (a.nsrc > 0) s.q_valid = 0; // the level-0 queue comes from a sweep of F_0
     double td_s = 0, bu_s = 0, gen_s = 0;
     long long pend_n = 0, pend_m = 0, n_f = 0;
     for (;;) {
@@ -368,8 +377,9 @@ static int bfs_host_loop(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs_
         st->t_bfs_s = ms * 1e-3;
         st->n_levels = L;
         st->max_level = pend_n > 0 ? L + 1 : (n_f > 0 ? L : L - 1);
-        st->v_cc = s.cum_n + pend_n;
+        st->v_cc = s.cum_n + pend_n + c->h_ctl->seed_iso;
         st->m_cc = s.cum_m + pend_m;
+        if (st->max_level < 0 && c->h_ctl->seed_iso > 0) st->max_level = 0; // isolated seeds only
         st->td_s = td_s;
         st->bu_s = bu_s;
         st->gen_s = gen_s;
@@ -418,12 +428,60 @@ static int bfs_persistent(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs
         st->t_bfs_s = ms * 1e-3;
         st->n_levels = h.n_levels;
         st->max_level = h.max_level;
-        st->v_cc = h.v_cc;
+        st->v_cc = h.v_cc + h.seed_iso;
         st->m_cc = h.m_cc;
+        if (st->max_level < 0 && h.seed_iso > 0) st->max_level = 0; // isolated seeds only
         st->td_s = td_s;
         st->bu_s = bu_s;
         st->gen_s = 0;
     }
+    return XBFS_OK;
+}
+
+extern "C" {
+
+} // extern "C"
+
+// Shared body of xbfs_bfs (nsrc == 0: single source src) and xbfs_bfs_multi
+// (nsrc seeds in srcs, host memory, already range-checked).
+static int bfs_run(xbfs_ctx *c, int32_t src, const int32_t *srcs, int32_t nsrc,
+                   const xbfs_opts *opts, int32_t *level_out, int32_t *parent_out,
+                   xbfs_level_rec *rec, int32_t rec_cap, xbfs_stats *st)
+{
+    if (rec_cap < 0 || (rec_cap > 0 && !rec)) return XBFS_ERR_ARG;
+    xbfs_opts o;
+    int rc = xb_resolve_opts(opts, &o);
+    if (rc) return rc;
+    if (parent_out) o.want_parent = 1;
+    XB_HIP(hipSetDevice(c->device));
+    if (o.policy == XBFS_POLICY_SCHEDULE)
+        XB_HIP(hipMemcpyAsync(c->d_sched, o.schedule, (size_t)o.schedule_len,
+                              hipMemcpyHostToDevice, c->stream));
+    XbArgs a = make_args(c, src, o);
+    if (nsrc > 0) {
+        if (nsrc > c->srcs_cap) {
+            if (c->d_srcs) XB_HIP(hipFree(c->d_srcs));
+            c->d_srcs = nullptr;
+            c->srcs_cap = 0;
+            if (hipMalloc(&c->d_srcs, sizeof(int32_t) * (size_t)nsrc) != hipSuccess)
+                return XBFS_ERR_NOMEM;
+            c->srcs_cap = nsrc;
+        }
+        XB_HIP(hipMemcpyAsync(c->d_srcs, srcs, sizeof(int32_t) * (size_t)nsrc,
+                              hipMemcpyHostToDevice, c->stream));
+        a.srcs = c->d_srcs;
+        a.nsrc = nsrc;
+        a.src = srcs[0];
+    }
+    rc = o.persistent ? bfs_persistent(c, a, o, rec, rec_cap, st)
+                      : bfs_host_loop(c, a, o, rec, rec_cap, st);
+    if (rc) return rc;
+    if (level_out)
+        XB_HIP(hipMemcpy(level_out, c->d_level, sizeof(int32_t) * (size_t)c->n,
+                         hipMemcpyDeviceToHost));
+    if (parent_out)
+        XB_HIP(hipMemcpy(parent_out, c->d_parent, sizeof(int32_t) * (size_t)c->n,
+                         hipMemcpyDeviceToHost));
     return XBFS_OK;
 }
 
@@ -434,25 +492,313 @@ int xbfs_bfs(xbfs_ctx *c, int32_t src, const xbfs_opts *opts, int32_t *level_out
 {
     if (!c) return XBFS_ERR_ARG;
     if (src < 0 || (int64_t)src >= c->n) return XBFS_ERR_ARG;
-    if (rec_cap < 0 || (rec_cap > 0 && !rec)) return XBFS_ERR_ARG;
+    return bfs_run(c, src, nullptr, 0, opts, level_out, parent_out, rec, rec_cap, st);
+}
+
+int xbfs_bfs_multi(xbfs_ctx *c, const int32_t *srcs, int32_t nsrc, const xbfs_opts *opts,
+                   int32_t *level_out, int32_t *parent_out, xbfs_level_rec *rec, int32_t rec_cap,
+                   xbfs_stats *st)
+{
+    if (!c || !srcs || nsrc <= 0) return XBFS_ERR_ARG;
+    for (int32_t i = 0; i < nsrc; i++)
+        if (srcs[i] < 0 || (int64_t)srcs[i] >= c->n) return XBFS_ERR_ARG;
+    return bfs_run(c, srcs[0], srcs, nsrc, opts, level_out, parent_out, rec, rec_cap, st);
+}
+
+int xbfs_bfs_multigraph(xbfs_ctx *c, const int64_t *vert_off, int32_t k, const int32_t *local_src,
+                        const xbfs_opts *opts, int32_t *level_out, xbfs_stats *st)
+{
+    if (!c || !vert_off || k <= 0 || !local_src) return XBFS_ERR_ARG;
+    if (vert_off[0] != 0 || vert_off[k] != c->n) return XBFS_ERR_ARG;
+    std::vector<int32_t> g((size_t)k);
+    for (int32_t i = 0; i < k; i++) {
+        if (vert_off[i + 1] < vert_off[i]) return XBFS_ERR_ARG;
+        if (local_src[i] < 0 || vert_off[i] + local_src[i] >= vert_off[i + 1]) return XBFS_ERR_ARG;
+        g[(size_t)i] = (int32_t)(vert_off[i] + local_src[i]);
+    }
+    return bfs_run(c, g[0], g.data(), k, opts, level_out, nullptr, nullptr, 0, st);
+}
+
+} // extern "C"
+
+// Median over reps runs of the objective of xbfs_autotune.
+static int tune_measure(xbfs_ctx *c, const int32_t *srcs, int32_t nsrc, int32_t multi,
+                        int32_t reps, const xbfs_opts &o, double *t_out)
+{
+    std::vector<double> ts((size_t)reps);
+    double total = 0.0;
+    const int32_t nrun = multi ? 1 : nsrc;
+    for (int32_t i = 0; i < nrun; i++) {
+        for (int32_t r = 0; r < reps; r++) {
+            xbfs_stats st;
+            const int rc = multi ? bfs_run(c, srcs[0], srcs, nsrc, &o, nullptr, nullptr, nullptr, 0, &st)
+                                 : bfs_run(c, srcs[i], nullptr, 0, &o, nullptr, nullptr, nullptr, 0, &st);
+            if (rc) return rc;
+            ts[(size_t)r] = st.t_bfs_s;
+        }
+        std::sort(ts.begin(), ts.end());
+        total += ts[(size_t)reps / 2];
+    }
+    *t_out = total;
+    return XBFS_OK;
+}
+
+extern "C" {
+
+int xbfs_autotune(xbfs_ctx *c, const int32_t *srcs, int32_t nsrc, int32_t multi, int32_t reps,
+                  xbfs_opts *opts, xbfs_tune_rec *trials, int32_t cap, int32_t *ntrials)
+{
+    if (!c || !srcs || nsrc <= 0 || !opts || reps <= 0 || cap < 0 || (cap > 0 && !trials))
+        return XBFS_ERR_ARG;
+    for (int32_t i = 0; i < nsrc; i++)
+        if (srcs[i] < 0 || (int64_t)srcs[i] >= c->n) return XBFS_ERR_ARG;
+    xbfs_opts best;
+    int rc = xb_resolve_opts(opts, &best);
+    if (rc) return rc;
+    const int32_t full = best.persistent ? c->coop_blocks : c->num_cu * 8;
+    if (best.grid_dim <= 0 || best.grid_dim > full) best.grid_dim = full;
+    const std::vector<int32_t> cand_t1 = {8, 16, 32, 64, 128};
+    const std::vector<int32_t> cand_t2 = {256, 512, 1024, 2048, 4096};
+    const std::vector<int32_t> cand_t3 = {4096, 16384, 65536, 262144};
+    std::vector<int32_t> cand_g;
+    for (int32_t f : {8, 6, 4})
+        if (full * f / 8 >= 1) cand_g.push_back(full * f / 8);
+    int32_t nt = 0;
+    auto record = [&](const xbfs_opts &o, double t) {
+        if (nt < cap) {
+            xbfs_tune_rec &r = trials[nt];
+            r.bin_thread_max = o.bin_thread_max;
+            r.bin_wave_max = o.bin_wave_max;
+            r.bin_block_max = o.bin_block_max;
+            r.grid_dim = o.grid_dim;
+            r.t_s = t;
+        }
+        nt++;
+    };
+    double t_best = 0.0;
+    rc = tune_measure(c, srcs, nsrc, multi, reps, best, &t_best);
+    if (rc) return rc;
+    record(best, t_best);
+    for (int pass = 0; pass < 2; pass++) {
+        for (int p = 0; p < 4; p++) {
+            const std::vector<int32_t> &cand = p == 0 ? cand_t1 : p == 1 ? cand_t2 : p == 2 ? cand_t3 : cand_g;
+            for (int32_t v : cand) {
+                xbfs_opts o = best;
+                int32_t *field = p == 0 ? &o.bin_thread_max : p == 1 ? &o.bin_wave_max
+                               : p == 2 ? &o.bin_block_max : &o.grid_dim;
+                if (*field == v) continue;
+                *field = v;
+                if (o.bin_wave_max < o.bin_thread_max || o.bin_block_max < o.bin_wave_max) continue;
+                double t = 0.0;
+                rc = tune_measure(c, srcs, nsrc, multi, reps, o, &t);
+                if (rc) return rc;
+                record(o, t);
+                if (t < t_best) {
+                    t_best = t;
+                    best = o;
+                }
+            }
+        }
+    }
+    *opts = best;
+    if (ntrials) *ntrials = std::min(nt, cap);
+    return XBFS_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// 64-source batch (kernels in xbfs_batch.hip)
+// ---------------------------------------------------------------------------
+#define XB_B64_HIST_BUDGET (16ll << 30) /* bytes of visited history at most */
+
+// Degrees, 32-edge segments and the batch buffers, once per context.
+static int b64_setup(xbfs_ctx *c)
+{
+    if (c->b64_ready) return XBFS_OK;
+    const int64_t n = c->n, m = c->m;
+    const int grid = c->num_cu * 8;
+    int32_t *cnt = nullptr;
+    long long *off = nullptr, *part = nullptr;
+    const int64_t T = ceil_div(n, XB_SCAN_CHUNK);
+    int rc = XBFS_OK;
+    if (hipMalloc(&c->d_b64_deg, sizeof(int32_t) * (size_t)n) != hipSuccess ||
+        hipMalloc(&cnt, sizeof(int32_t) * (size_t)n) != hipSuccess ||
+        hipMalloc(&off, sizeof(long long) * (size_t)n) != hipSuccess ||
+        hipMalloc(&part, sizeof(long long) * (size_t)T) != hipSuccess)
+        rc = XBFS_ERR_NOMEM;
+    int64_t nseg = 0;
+    if (rc == XBFS_OK) {
+        hipLaunchKernelGGL(k_b64_deg, dim3(grid), dim3(XB_B64_BLOCK), 0, c->stream, c->d_beg, n,
+                           c->d_b64_deg, cnt);
+        hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)T), dim3(256), 0, c->stream, cnt, n, part);
+        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(1024), 0, c->stream, part, T);
+        hipLaunchKernelGGL(k_scan_down, dim3((unsigned)T), dim3(256), 0, c->stream, cnt, n, part, off);
+        long long last_off = 0;
+        int32_t last_cnt = 0;
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(&last_off, off + (n - 1), sizeof(long long), hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess ||
+            hipMemcpyAsync(&last_cnt, cnt + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = XBFS_ERR_HIP;
+        nseg = last_off + last_cnt;
+    }
+    if (rc == XBFS_OK && (hipMalloc(&c->d_b64_segu, sizeof(int32_t) * (size_t)std::max<int64_t>(nseg, 1)) != hipSuccess ||
+                          hipMalloc(&c->d_b64_sege, sizeof(int64_t) * (size_t)(nseg + 1)) != hipSuccess))
+        rc = XBFS_ERR_NOMEM;
+    if (rc == XBFS_OK) {
+        hipLaunchKernelGGL(k_b64_seg_fill, dim3(grid), dim3(XB_B64_BLOCK), 0, c->stream, c->d_beg, n,
+                           cnt, off, c->d_b64_segu, c->d_b64_sege, nseg, m);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = XBFS_ERR_HIP;
+    }
+    if (cnt) (void)hipFree(cnt);
+    if (off) (void)hipFree(off);
+    if (part) (void)hipFree(part);
+    if (rc) return rc;
+    c->b64_nseg = nseg;
+    const int64_t H = std::max<int64_t>(8, std::min<int64_t>(XB_B64_HMAX, XB_B64_HIST_BUDGET / (8 * n)));
+    c->b64_H = (int32_t)H;
+    if (hipMalloc(&c->d_b64_F, sizeof(uint64_t) * (size_t)n * 2) != hipSuccess ||
+        hipMalloc(&c->d_b64_acc, sizeof(uint64_t) * (size_t)n) != hipSuccess ||
+        hipMalloc(&c->d_b64_hist, sizeof(uint64_t) * (size_t)n * (size_t)H) != hipSuccess ||
+        hipMalloc(&c->d_b64_lv, sizeof(int32_t) * (size_t)n * 64) != hipSuccess ||
+        hipMalloc(&c->d_b64_out, sizeof(long long) * 8) != hipSuccess ||
+        hipMalloc(&c->d_b64_srcs, sizeof(int32_t) * 64) != hipSuccess)
+        return XBFS_ERR_NOMEM;
+    for (hipEvent_t &e : c->ev_b64)
+        if (!e) XB_HIP(hipEventCreate(&e));
+    c->b64_ready = 1;
+    return XBFS_OK;
+}
+
+// Direction of batch level L: 0 push, 1 pull.
+static int b64_mode(const xbfs_opts &o, int32_t L, long long m_f, long long m_u)
+{
+    switch (o.policy) {
+    case XBFS_POLICY_FORCE_TD_SF:
+    case XBFS_POLICY_FORCE_TD_SS: return 0;
+    case XBFS_POLICY_FORCE_BU: return 1;
+    case XBFS_POLICY_SCHEDULE:
+        if (L < o.schedule_len) return o.schedule[L] == XBFS_MODE_BU ? 1 : 0;
+        break;
+    default: break;
+    }
+    // pull once the frontier's edges exceed alpha x the edges of vertices
+    // that still miss a source
+    return (double)m_f > (double)o.alpha * (double)m_u ? 1 : 0;
+}
+
+extern "C" {
+
+int xbfs_bfs_batch64(xbfs_ctx *c, const int32_t *srcs, int32_t nsrc, const xbfs_opts *opts,
+                     int32_t *levels_out, xbfs_stats *st)
+{
+    if (!c || !srcs || nsrc <= 0 || nsrc > 64 || c->n <= 0) return XBFS_ERR_ARG;
+    for (int32_t i = 0; i < nsrc; i++)
+        if (srcs[i] < 0 || (int64_t)srcs[i] >= c->n) return XBFS_ERR_ARG;
     xbfs_opts o;
     int rc = xb_resolve_opts(opts, &o);
     if (rc) return rc;
-    if (parent_out) o.want_parent = 1;
     XB_HIP(hipSetDevice(c->device));
-    if (o.policy == XBFS_POLICY_SCHEDULE)
-        XB_HIP(hipMemcpyAsync(c->d_sched, o.schedule, (size_t)o.schedule_len,
-                              hipMemcpyHostToDevice, c->stream));
-    const XbArgs a = make_args(c, src, o);
-    rc = o.persistent ? bfs_persistent(c, a, o, rec, rec_cap, st)
-                      : bfs_host_loop(c, a, o, rec, rec_cap, st);
+    rc = b64_setup(c);
     if (rc) return rc;
-    if (level_out)
-        XB_HIP(hipMemcpy(level_out, c->d_level, sizeof(int32_t) * (size_t)c->n,
+    const int64_t n = c->n;
+    const int grid = c->num_cu * 8;
+    const uint64_t all = nsrc == 64 ? ~0ull : ((1ull << nsrc) - 1ull);
+    long long h_out[8] = {0};
+    XbB64Args a;
+    a.deg = c->d_b64_deg;
+    a.adj = c->d_adj;
+    a.seg_u = c->d_b64_segu;
+    a.seg_e = c->d_b64_sege;
+    a.nseg = c->b64_nseg;
+    a.acc = c->d_b64_acc;
+    a.part = c->d_part;
+    a.pstride = XB_PMAX;
+    a.n = n;
+    a.all = all;
+    XB_HIP(hipEventRecord(c->ev_b64[0], c->stream));
+    XB_HIP(hipMemcpyAsync(c->d_b64_srcs, srcs, sizeof(int32_t) * (size_t)nsrc,
+                          hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_b64_clear, dim3(grid), dim3(XB_B64_BLOCK), 0, c->stream, c->d_b64_hist,
+                       c->d_b64_F, c->d_b64_acc, n);
+    hipLaunchKernelGGL(k_b64_seed, dim3(1), dim3(64), 0, c->stream, c->d_b64_srcs, nsrc,
+                       c->d_b64_deg, c->d_b64_hist, c->d_b64_F, c->d_b64_out);
+    XB_LAUNCHED();
+    XB_HIP(hipMemcpyAsync(h_out, c->d_b64_out, sizeof(long long) * 2, hipMemcpyDeviceToHost,
+                          c->stream));
+    XB_HIP(hipStreamSynchronize(c->stream));
+    long long n_f = h_out[0], m_f = h_out[1], m_u = c->m;
+    uint64_t active = all;
+    int32_t L = 0, max_level = 0;
+    const int32_t H = c->b64_H;
+    while (n_f > 0 && L + 1 < H) {
+        const int mode = b64_mode(o, L, m_f, m_u);
+        a.F_cur = c->d_b64_F + (size_t)(L & 1) * (size_t)n;
+        a.F_next = c->d_b64_F + (size_t)((L + 1) & 1) * (size_t)n;
+        a.vis_cur = c->d_b64_hist + (size_t)L * (size_t)n;
+        a.vis_next = c->d_b64_hist + (size_t)(L + 1) * (size_t)n;
+        if (mode) hipLaunchKernelGGL(k_b64_pull, dim3(grid), dim3(XB_B64_BLOCK), 0, c->stream, a);
+        else hipLaunchKernelGGL(k_b64_push, dim3(grid), dim3(XB_B64_BLOCK), 0, c->stream, a);
+        hipLaunchKernelGGL(k_b64_fin, dim3(grid), dim3(XB_B64_BLOCK), 0, c->stream, a);
+        hipLaunchKernelGGL(k_b64_reduce, dim3(1), dim3(1024), 0, c->stream, c->d_part,
+                           (int64_t)XB_PMAX, grid, XB_B64_F_NF, 4, XB_B64_F_OR, c->d_b64_out);
+        XB_LAUNCHED();
+        XB_HIP(hipMemcpyAsync(h_out, c->d_b64_out, sizeof(long long) * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+        XB_HIP(hipStreamSynchronize(c->stream));
+        n_f = h_out[0];
+        m_f = h_out[1];
+        m_u = h_out[2];
+        active = (uint64_t)h_out[3];
+        L++;
+        if (n_f > 0) max_level = L;
+    }
+    const int32_t nslots = L + 1;
+    const uint64_t unfinished = n_f > 0 ? (active & all) : 0ull;
+    const uint64_t done = all & ~unfinished;
+    hipLaunchKernelGGL(k_b64_flush, dim3(grid), dim3(XB_B64_BLOCK), 0, c->stream, c->d_b64_hist,
+                       nslots, n, nsrc, done, c->d_b64_deg, c->d_b64_lv, c->d_part,
+                       (int64_t)XB_PMAX);
+    hipLaunchKernelGGL(k_b64_reduce, dim3(1), dim3(1024), 0, c->stream, c->d_part,
+                       (int64_t)XB_PMAX, grid, XB_B64_F_VC, 2, -1, c->d_b64_out);
+    XB_LAUNCHED();
+    XB_HIP(hipMemcpyAsync(h_out, c->d_b64_out, sizeof(long long) * 2, hipMemcpyDeviceToHost,
+                          c->stream));
+    XB_HIP(hipStreamSynchronize(c->stream));
+    long long v_cc = h_out[0], m_cc = h_out[1];
+    // sources whose traversal outlasts the history window: single-source engine
+    for (int32_t s = 0; s < nsrc; s++) {
+        if (!((unfinished >> s) & 1ull)) continue;
+        xbfs_opts so;
+        xbfs_opts_default(&so);
+        xbfs_stats s1;
+        rc = bfs_run(c, srcs[s], nullptr, 0, &so, nullptr, nullptr, nullptr, 0, &s1);
+        if (rc) return rc;
+        XB_HIP(hipMemcpyAsync(c->d_b64_lv + (size_t)s * (size_t)n, c->d_level,
+                              sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+        v_cc += s1.v_cc;
+        m_cc += s1.m_cc;
+        max_level = std::max(max_level, s1.max_level);
+    }
+    XB_HIP(hipEventRecord(c->ev_b64[1], c->stream));
+    XB_HIP(hipEventSynchronize(c->ev_b64[1]));
+    float ms = 0.0f;
+    XB_HIP(hipEventElapsedTime(&ms, c->ev_b64[0], c->ev_b64[1]));
+    if (levels_out)
+        XB_HIP(hipMemcpy(levels_out, c->d_b64_lv, sizeof(int32_t) * (size_t)n * (size_t)nsrc,
                          hipMemcpyDeviceToHost));
-    if (parent_out)
-        XB_HIP(hipMemcpy(parent_out, c->d_parent, sizeof(int32_t) * (size_t)c->n,
-                         hipMemcpyDeviceToHost));
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->t_bfs_s = ms * 1e-3;
+        st->n_levels = L;
+        st->max_level = max_level;
+        st->v_cc = v_cc;
+        st->m_cc = m_cc;
+    }
     return XBFS_OK;
 }
 
@@ -587,9 +933,11 @@ extern "C" int xbfs_bfs_probe(xbfs_ctx *c, int32_t src, const xbfs_opts *opts, i
     const int grid = o.grid_dim > 0 ? o.grid_dim : c->num_cu * 8;
     hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl);
     hipLaunchKernelGGL(k_ph_init, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a);
+    if (a.nsrc > 0) hipLaunchKernelGGL(k_ph_seed, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a);
     XB_LAUNCHED();
     XbLoop s;
     xb_loop_init(&s);
+    if (a.nsrc > 0) s.q_valid = 0; // the level-0 queue comes from a sweep of F_0
     long long pend_n = 0, pend_m = 0, n_f = 0;
     double t_total = 0;
     std::vector<float> tr((size_t)reps);
@@ -674,7 +1022,7 @@ extern "C" int xbfs_bfs_probe(xbfs_ctx *c, int32_t src, const xbfs_opts *opts, i
         st->t_bfs_s = t_total; // sum of the executed modes' probed times
         st->n_levels = L;
         st->max_level = pend_n > 0 ? L + 1 : (n_f > 0 ? L : L - 1);
-        st->v_cc = s.cum_n + pend_n;
+        st->v_cc = s.cum_n + pend_n + c->h_ctl->seed_iso;
         st->m_cc = s.cum_m + pend_m;
         st->td_s = st->bu_s = st->gen_s = 0;
     }

@@ -50,6 +50,23 @@ struct xbfs_ctx {
     uint64_t *s_vis = nullptr, *s_fbm = nullptr;
     int32_t *s_qbuf = nullptr;
     XbCtl *s_ctl = nullptr;
+    // multi-source seeds (grown on demand)
+    int32_t *d_srcs = nullptr;
+    int64_t srcs_cap = 0;
+    // 64-source batch engine (allocated on the first xbfs_bfs_batch64)
+    int b64_ready = 0;
+    int32_t b64_H = 0;             // history slots
+    int64_t b64_nseg = 0;          // 32-edge segments
+    int32_t *d_b64_deg = nullptr;  // int32[n]
+    int32_t *d_b64_segu = nullptr; // int32[nseg]
+    int64_t *d_b64_sege = nullptr; // int64[nseg+1]
+    uint64_t *d_b64_F = nullptr;   // 2 x uint64[n]
+    uint64_t *d_b64_acc = nullptr; // uint64[n]
+    uint64_t *d_b64_hist = nullptr;// H x uint64[n]
+    int32_t *d_b64_lv = nullptr;   // 64 x int32[n]
+    long long *d_b64_out = nullptr;// 8 reduced statistics
+    int32_t *d_b64_srcs = nullptr; // 64 sources
+    hipEvent_t ev_b64[2] = {nullptr, nullptr};
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::vector<hipEvent_t> ev_pool;
 };

@@ -35,6 +35,8 @@ struct alignas(256) XbCtl {
     XbCounter qcnt[XB_QBUFS][XB_BINS];  // queue lengths, one cache line each
     int32_t n_levels, max_level, pad0, pad1;
     long long v_cc, m_cc;
+    long long seed_iso;                 // multi-source: distinct degree-0 seeds (level 0, no edges)
+    long long pad2;
     long long sum[16];                  // host-driven loop: reduced statistics (k_part_reduce)
 };
 
@@ -74,6 +76,8 @@ struct XbArgs {
     const float *mlp;   // learned policy weights (NULL = none)
     int64_t n, m, nwords, ntiles, n_nz;
     int64_t fstride; // words between frontier rings: nwords rounded up to a tile (256 B)
+    const int32_t *srcs; // multi-source seeds (device), used when nsrc > 0
+    int32_t nsrc;
     int32_t src, t1, t2, t3;
     int32_t policy;
     float alpha;

@@ -153,11 +153,12 @@ __device__ void ph_init(const XbArgs &a, int bid, int nb)
     const int64_t n4 = a.n >> 2;
     int4 *l4 = reinterpret_cast<int4 *>(a.level);
     int4 *p4 = reinterpret_cast<int4 *>(a.parent);
-    const int32_t s = a.src;
+    // single source: seeded here; multi-source: ph_seed seeds after the barrier
+    const int32_t s = a.nsrc > 0 ? -1 : a.src;
     for (int64_t i = gtid; i < n4; i += nthr) {
         int4 v = make_int4(-1, -1, -1, -1);
         int4 p = v;
-        if ((s >> 2) == i) {
+        if (s >= 0 && (s >> 2) == i) {
             int k = s & 3;
             if (k == 0) { v.x = 0; p.x = s; }
             else if (k == 1) { v.y = 0; p.y = s; }
@@ -171,8 +172,8 @@ __device__ void ph_init(const XbArgs &a, int bid, int nb)
         a.level[i] = (i == s) ? 0 : -1;
         if (a.parent) a.parent[i] = (i == s) ? s : -1;
     }
-    const uint64_t sbit = 1ull << (s & 63);
-    const int64_t sw = s >> 6;
+    const uint64_t sbit = s >= 0 ? 1ull << (s & 63) : 0ull;
+    const int64_t sw = s >= 0 ? (s >> 6) : -1;
     for (int64_t w = gtid; w < a.nwords; w += nthr) {
         const uint64_t v = (w == sw) ? sbit : 0ull;
         a.vis[w] = v;
@@ -182,10 +183,10 @@ __device__ void ph_init(const XbArgs &a, int bid, int nb)
         xb_f(a, 3)[w] = 0ull;
     }
     // partial statistics: all zero except level 0's slot = (1, deg(src))
-    const long long dsrc = a.beg[s + 1] - a.beg[s];
+    const long long dsrc = s >= 0 ? a.beg[s + 1] - a.beg[s] : 0;
     for (int64_t i = gtid; i < (int64_t)XB_NFIELDS * XB_PMAX; i += nthr) {
         long long v = 0;
-        if (i == (int64_t)xb_f_ring(0, 0) * XB_PMAX) v = 1;
+        if (s >= 0 && i == (int64_t)xb_f_ring(0, 0) * XB_PMAX) v = 1;
         else if (i == (int64_t)xb_f_ring(0, 1) * XB_PMAX) v = dsrc;
         a.part[i] = v;
     }
@@ -197,12 +198,47 @@ __device__ void ph_init(const XbArgs &a, int bid, int nb)
         xb_st(&c->max_level, 0);
         xb_st(&c->v_cc, 0ll);
         xb_st(&c->m_cc, 0ll);
+        xb_st(&c->seed_iso, 0ll);
         const int bin = dsrc <= a.t1 ? 0 : (dsrc <= a.t2 ? 1 : (dsrc <= a.t3 ? 2 : 3));
         if (dsrc > 0) {
             xb_q(a, 0, bin)[0] = s;
             xb_st(&c->qcnt[0][bin].v, 1);
         }
     }
+}
+
+// Multi-source seeding (after ph_init and a barrier): every distinct seed
+// gets level 0 and its visited bit; seeds with edges also get their F_0 bit
+// and count toward level 0's statistics.  Degree-0 seeds are counted apart
+// (they are reached but are not frontier vertices).  Duplicates are resolved
+// by the visited bit: only the lane that sets it writes the seed.  The
+// level-0 queue is then built by a sweep of F_0.
+__device__ void ph_seed(const XbArgs &a, int bid, int nb)
+{
+    const int64_t gtid = (int64_t)bid * blockDim.x + threadIdx.x;
+    const int64_t nthr = (int64_t)nb * blockDim.x;
+    unsigned long long nv = 0, mv = 0, iso = 0;
+    for (int64_t i = gtid; i < a.nsrc; i += nthr) {
+        const int32_t s = a.srcs[i];
+        const unsigned long long bit = 1ull << (s & 63);
+        const unsigned long long old =
+            atomicOr(reinterpret_cast<unsigned long long *>(&a.vis[s >> 6]), bit);
+        if (old & bit) continue;
+        a.level[s] = 0;
+        if (a.parent) a.parent[s] = s;
+        const long long d = a.beg[s + 1] - a.beg[s];
+        if (d > 0) {
+            atomicOr(reinterpret_cast<unsigned long long *>(&xb_f(a, 0)[s >> 6]), bit);
+            nv += 1;
+            mv += (unsigned long long)d;
+        } else {
+            iso += 1;
+        }
+    }
+    if (iso) atomicAdd(reinterpret_cast<unsigned long long *>(&a.ctl->seed_iso), iso);
+    unsigned long long vv[2] = {nv, mv};
+    const int ff[2] = {xb_f_ring(0, 0), xb_f_ring(0, 1)};
+    xb_part_add<2>(a, bid, vv, ff);
 }
 
 // Zeroing schedule at the start of level L (everything zeroed here is
@@ -669,8 +705,13 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
     XbWaveLds *lds = reinterpret_cast<XbWaveLds *>(xb_dyn_lds);
     ph_init(a, bid, nb);
     if (!xb_grid_sync(a.ctl)) return;
+    if (a.nsrc > 0) {
+        ph_seed(a, bid, nb);
+        if (!xb_grid_sync(a.ctl)) return;
+    }
     XbLoop st;
     xb_loop_init(&st);
+    if (a.nsrc > 0) st.q_valid = 0; // the level-0 queue comes from a sweep of F_0
     XbCtl *c = a.ctl;
     for (;;) {
         const int32_t L = st.L;
@@ -768,6 +809,7 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_coop_level(XbArgs a, int32
 
 // ---- per-phase kernels for the host-driven loop ---------------------------
 __global__ void __launch_bounds__(XB_PBLOCK) k_ph_init(XbArgs a) { ph_init(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(XB_PBLOCK) k_ph_seed(XbArgs a) { ph_seed(a, blockIdx.x, gridDim.x); }
 
 // from_levels != 0: the sweep after a single-scan level (builds the bits too)
 __global__ void __launch_bounds__(XB_PBLOCK) k_ph_sweep(XbArgs a, int32_t L, int32_t from_levels,
