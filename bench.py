@@ -23,6 +23,14 @@ by the device time of the launch that runs the traversal, against the
 
 cpu_baseline: OpenMP top-down BFS (the oracle's CPU baseline) on a bounded
 sample of the same sources, on this host's cores.
+
+batch64 (reported beside the headline, --batch 1): RMAT scale 22 ef16 (seed
+1022), 64 seeded sources; device time of one 64-source bit-parallel batch
+against the sum of the 64 single-source traversals of the same sources
+(medians of 3).  multigraph: 256 RMAT graphs (scale U{10..16}, edge factor
+U{4..32}, seeds 10000+i), one seeded source each; one multi-source traversal
+of the packed graphs against 256 single-source traversals on per-graph
+contexts, with the autotuner's setting for the packed graph.
 """
 import argparse
 import json
@@ -58,6 +66,64 @@ def pick_sources(deg, k, seed=42):
     return rng.choice(cand, size=min(k, cand.size), replace=False).astype(np.int32)
 
 
+def batch_fields(reps=3):
+    """The two batched workloads; returns (batch64 dict, multigraph dict)."""
+    from xbfs_amd import Context, Graph
+    g = Graph.rmat(22, 16, 0.57, 0.19, 0.19, seed=1022)
+    ctx = Context(g, device=0)
+    srcs = pick_sources(g.degrees(), 64)
+    ctx.bfs_batch64(srcs, want_levels=False)
+    seq = 0.0
+    for s in srcs:
+        seq += float(np.median([ctx.bfs(int(s), want_levels=False)[3]["t_bfs_s"]
+                                for _ in range(reps)]))
+    tb, st = [], None
+    for _ in range(reps):
+        _, st = ctx.bfs_batch64(srcs, want_levels=False)
+        tb.append(st["t_bfs_s"])
+    tb = float(np.median(tb))
+    b64 = {"workload": "RMAT scale 22 ef16 seed 1022, 64 seeded sources", "n": g.n, "m": g.m,
+           "batch_ms": round(tb * 1e3, 3), "sequential_ms": round(seq * 1e3, 3),
+           "speedup": round(seq / tb, 3), "aggregate_gteps": round(st["m_cc"] / tb / 1e9, 1),
+           "levels": st["n_levels"]}
+    ctx.close()
+    del g, ctx
+
+    gs, ls = [], []
+    for i in range(256):
+        rng = np.random.default_rng(10000 + i)
+        scale = int(rng.integers(10, 17))
+        ef = int(rng.integers(4, 33))
+        gi = Graph.rmat(scale, ef, 0.57, 0.19, 0.19, seed=10000 + i)
+        gs.append(gi)
+        ls.append(int(pick_sources(gi.degrees(), 1, seed=10000 + i)[0]))
+    seq = 0.0
+    for gi, s in zip(gs, ls):
+        ci = Context(gi, device=0)
+        ci.bfs(s, want_levels=False)
+        seq += float(np.median([ci.bfs(s, want_levels=False)[3]["t_bfs_s"] for _ in range(reps)]))
+        ci.close()
+    u, off = Graph.concat(gs)
+    cu = Context(u, device=0)
+    seeds = (off[:-1] + np.asarray(ls)).astype(np.int32)
+    tuned, trials = cu.autotune(seeds, multi=True, reps=reps)
+    tm, st = [], None
+    for _ in range(reps):
+        _, st = cu.bfs_multigraph(off, ls, opts=tuned, want_levels=False)
+        tm.append(st["t_bfs_s"])
+    tm = float(np.median(tm))
+    t_default = trials[0]["t_s"]
+    mg = {"workload": "256 RMAT graphs, scale U{10..16}, ef U{4..32}, seeds 10000+i, one source each",
+          "n": u.n, "m": u.m, "batch_ms": round(tm * 1e3, 3), "sequential_ms": round(seq * 1e3, 3),
+          "speedup": round(seq / tm, 3), "aggregate_gteps": round(st["m_cc"] / tm / 1e9, 1),
+          "levels": st["n_levels"], "autotune": {
+              "bin_thread_max": tuned.bin_thread_max, "bin_wave_max": tuned.bin_wave_max,
+              "bin_block_max": tuned.bin_block_max, "grid_dim": tuned.grid_dim,
+              "trials": len(trials), "default_ms": round(t_default * 1e3, 3)}}
+    cu.close()
+    return b64, mg
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +142,8 @@ def main():
     ap.add_argument("--compare", type=int, default=1,
                     help="1 = also time the heuristic policy; 0 = only the main policy "
                          "(profiling runs, so every k_persistent launch is the measured one)")
+    ap.add_argument("--batch", type=int, default=1,
+                    help="1 = also measure the 64-source batch and the 256-graph multigraph batch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="run only a plain device-to-device copy of BYTES bytes, 5 times "
                          "(scale factor for the FETCH_SIZE / WRITE_SIZE counters), and exit")
@@ -186,6 +254,12 @@ def main():
         cpu = {"value": hmean(cg), "unit": "GTEPS", "cores": nthr, "kind": "port",
                "sample": f"OpenMP top-down BFS, first {len(cg)} of the 64 sources, same graph"}
 
+    b64 = mg = None
+    if args.batch:
+        b64, mg = batch_fields()
+        print(f"[bench] batch64 {b64['speedup']}x, multigraph {mg['speedup']}x", file=sys.stderr,
+              flush=True)
+
     out = {
         "metric": "GTEPS (harm. mean, 64 src) RMAT-24 ef16; learned vs heuristic switch; % HBM",
         "value": round(value, 3),
@@ -216,6 +290,8 @@ def main():
                      "launches_per_bfs": 1 if opts.persistent else None,
                      "avg_launch_us": round(float(t_dev[main_pol].mean()) * 1e6, 2)},
         "cpu_baseline": cpu,
+        "batch64": b64,
+        "multigraph": mg,
         "gteps_min": round(float(gteps.min()), 3),
         "gteps_max": round(float(gteps.max()), 3),
         "graph500_gteps_hmean": round(value / 2, 3),

@@ -594,19 +594,42 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
             wl->va[ln] = 0ull;
         }
         xb_wave_sync();
+        int ndef = 0; // deferred candidates, stored at the front of the list
         for (int r = 0; r < total; r += 64) {
             const int i = r + ln;
+            bool defer = false;
+            int32_t v = 0;
             if (i < total) {
-                const int32_t v = wl->list[i];
+                v = wl->list[i];
                 const int64_t b = a.beg[v], e = a.beg[v + 1];
+                // at most XB_BU_LANE_MAX edges per lane (4 neighbours per step,
+                // independent loads); longer scans are finished by the wave below
+                const int64_t ce = (e - b > XB_BU_LANE_MAX) ? b + XB_BU_LANE_MAX : e;
                 int32_t found = -1, seen = -1;
-                for (int64_t j = b; j < e; j++) {
-                    const int32_t u = a.adj[j];
-                    const uint64_t fw = F[u >> 6];
-                    if ((fw >> (u & 63)) & 1ull) { found = u; break; }
+                int64_t j = b;
+                for (; j + 4 <= ce; j += 4) {
+                    const int32_t u0 = a.adj[j], u1 = a.adj[j + 1], u2 = a.adj[j + 2],
+                                  u3 = a.adj[j + 3];
+                    const bool h0 = (F[u0 >> 6] >> (u0 & 63)) & 1ull;
+                    const bool h1 = (F[u1 >> 6] >> (u1 & 63)) & 1ull;
+                    const bool h2 = (F[u2 >> 6] >> (u2 & 63)) & 1ull;
+                    const bool h3 = (F[u3 >> 6] >> (u3 & 63)) & 1ull;
+                    if (h0 | h1 | h2 | h3) {
+                        found = h0 ? u0 : (h1 ? u1 : (h2 ? u2 : u3));
+                        break;
+                    }
                     if (seen < 0) {
-                        const uint64_t nw = F1[u >> 6];
-                        if ((nw >> (u & 63)) & 1ull) seen = u;
+                        if ((F1[u0 >> 6] >> (u0 & 63)) & 1ull) seen = u0;
+                        else if ((F1[u1 >> 6] >> (u1 & 63)) & 1ull) seen = u1;
+                        else if ((F1[u2 >> 6] >> (u2 & 63)) & 1ull) seen = u2;
+                        else if ((F1[u3 >> 6] >> (u3 & 63)) & 1ull) seen = u3;
+                    }
+                }
+                if (found < 0) {
+                    for (; j < ce; j++) {
+                        const int32_t u = a.adj[j];
+                        if ((F[u >> 6] >> (u & 63)) & 1ull) { found = u; break; }
+                        if (seen < 0 && ((F1[u >> 6] >> (u & 63)) & 1ull)) seen = u;
                     }
                 }
                 const unsigned long long bit = 1ull << (v & 63);
@@ -618,6 +641,8 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                     atomicOr(&wl->va[li], bit);
                     n1 += 1;
                     m1 += (unsigned long long)(e - b);
+                } else if (ce < e) {
+                    defer = true; // unscanned edges remain: no +2 decision yet
                 } else if (seen >= 0) {
                     a.level[v] = L2;
                     if (a.parent) a.parent[v] = seen;
@@ -626,6 +651,52 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                     n2 += 1;
                     m2 += (unsigned long long)(e - b);
                 }
+            }
+            // deferred vertices go to list[0, ndef): every lane has read its
+            // entry of this round, and later rounds read at indices >= r + 64
+            const unsigned long long dm = __ballot(defer);
+            xb_wave_sync();
+            if (defer) wl->list[ndef + xb_rank(dm)] = v;
+            ndef += (int)__popcll(dm);
+            xb_wave_sync();
+        }
+        // The deferred candidates, one at a time by the whole wave: 256 edges
+        // per round (4 coalesced adjacency words per lane), a ballot ends the
+        // scan at the first level-L neighbour.  The +2 rule is not applied to
+        // them: one left unvisited is found at the next level.
+        for (int d = 0; d < ndef; d++) {
+            const int32_t v = wl->list[d];
+            const int64_t b = a.beg[v], e = a.beg[v + 1];
+            int32_t found = -1;
+            for (int64_t base = b + XB_BU_LANE_MAX; base < e; base += 256) {
+                int32_t u[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int64_t jj = base + ln + 64 * t;
+                    u[t] = jj < e ? a.adj[jj] : -1;
+                }
+                bool h[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) h[t] = u[t] >= 0 && ((F[u[t] >> 6] >> (u[t] & 63)) & 1ull);
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const unsigned long long m = __ballot(h[t]);
+                    if (m) {
+                        found = __shfl(u[t], __ffsll(m) - 1);
+                        break;
+                    }
+                }
+                if (found >= 0) break;
+            }
+            if (found >= 0 && ln == 0) {
+                const unsigned long long bit = 1ull << (v & 63);
+                const int li = (int)((v >> 6) - w0);
+                a.level[v] = L1;
+                if (a.parent) a.parent[v] = found;
+                atomicOr(&wl->f1[li], bit);
+                atomicOr(&wl->va[li], bit);
+                n1 += 1;
+                m1 += (unsigned long long)(e - b);
             }
         }
         xb_wave_sync();
@@ -697,10 +768,20 @@ __device__ void xb_mlp_block_forward(const float *p, const float *x, float *s_h1
     __syncthreads();
 }
 
+// The level loop's block-uniform state lives in LDS (thread 0 updates it,
+// everyone reads it after a barrier) so it does not hold scalar registers
+// across the phases, which are register-bound.
+struct XbLoopShared {
+    XbLoop st;
+    long long n_f, m_f, n_u, m_u, p2_next;
+    int32_t open, mode, htag, used;
+};
+
 __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
 {
     const int bid = blockIdx.x, nb = gridDim.x;
     __shared__ XbSmem sm;
+    __shared__ XbLoopShared ls;
     extern __shared__ __align__(16) unsigned char xb_dyn_lds[];
     XbWaveLds *lds = reinterpret_cast<XbWaveLds *>(xb_dyn_lds);
     ph_init(a, bid, nb);
@@ -709,73 +790,96 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
         ph_seed(a, bid, nb);
         if (!xb_grid_sync(a.ctl)) return;
     }
-    XbLoop st;
-    xb_loop_init(&st);
-    if (a.nsrc > 0) st.q_valid = 0; // the level-0 queue comes from a sweep of F_0
+    if (threadIdx.x == 0) {
+        xb_loop_init(&ls.st);
+        if (a.nsrc > 0) ls.st.q_valid = 0; // the level-0 queue comes from a sweep of F_0
+    }
+    __syncthreads();
     XbCtl *c = a.ctl;
     for (;;) {
-        const int32_t L = st.L;
+        const int32_t L = ls.st.L;
         const unsigned long long t0 = wall_clock64();
-        if (st.need_sweep) {
+        const bool need_sweep = ls.st.need_sweep != 0;
+        if (need_sweep) {
             ph_sweep(a, L, bid, nb, true, true, lds);
             if (!xb_grid_sync(c)) return;
         }
-        long long sv[5];
         {
-            const int ff[5] = {st.need_sweep ? xb_f_sweep(L, 0) : xb_f_ring(L, 0),
-                               st.need_sweep ? xb_f_sweep(L, 1) : xb_f_ring(L, 1),
+            long long sv[5];
+            const int ff[5] = {need_sweep ? xb_f_sweep(L, 0) : xb_f_ring(L, 0),
+                               need_sweep ? xb_f_sweep(L, 1) : xb_f_ring(L, 1),
                                xb_f_ring(L + 1, 0), xb_f_ring(L + 1, 1), xb_f_ring(L + 1, 2)};
             xb_part_sum<5>(a, ff, sv);
-        }
-        const long long n_f = sv[0], m_f = sv[1], pend_n = sv[2], pend_m = sv[3], p2_next = sv[4];
-        long long n_u = 0, m_u = 0;
-        const bool open = xb_level_open(st, a, n_f, m_f, pend_n, pend_m, &n_u, &m_u);
-        if (!open || L >= a.max_levels) {
-            if (bid == 0 && threadIdx.x == 0) {
-                xb_st(&c->n_levels, L);
-                xb_st(&c->max_level, pend_n > 0 ? L + 1 : (n_f > 0 ? L : L - 1));
-                xb_st(&c->v_cc, (long long)(st.cum_n + pend_n));
-                xb_st(&c->m_cc, (long long)(st.cum_m + pend_m));
-                if (open) xb_st(&c->error, 2);
+            if (threadIdx.x == 0) {
+                long long n_u = 0, m_u = 0;
+                const bool open = xb_level_open(ls.st, a, sv[0], sv[1], sv[2], sv[3], &n_u, &m_u);
+                ls.open = open ? 1 : 0;
+                ls.n_f = sv[0];
+                ls.m_f = sv[1];
+                ls.n_u = n_u;
+                ls.m_u = m_u;
+                ls.p2_next = sv[4];
+                if (!open || L >= a.max_levels) {
+                    if (bid == 0) {
+                        const long long pend_n = sv[2], pend_m = sv[3];
+                        xb_st(&c->n_levels, L);
+                        xb_st(&c->max_level, pend_n > 0 ? L + 1 : (sv[0] > 0 ? L : L - 1));
+                        xb_st(&c->v_cc, (long long)(ls.st.cum_n + pend_n));
+                        xb_st(&c->m_cc, (long long)(ls.st.cum_m + pend_m));
+                        if (open) xb_st(&c->error, 2);
+                    }
+                    ls.open = 0;
+                }
             }
-            return;
+            __syncthreads();
         }
+        if (!ls.open) return;
         const float *pred = nullptr;
         if (a.mlp) {
-            if (threadIdx.x == 0) xbfs_mlp_features(L, n_f, m_f, n_u, m_u, st.prev_mode, sm.mlp.x);
+            if (threadIdx.x == 0)
+                xbfs_mlp_features(L, ls.n_f, ls.m_f, ls.n_u, ls.m_u, ls.st.prev_mode, sm.mlp.x);
             __syncthreads();
             xb_mlp_block_forward(a.mlp, sm.mlp.x, sm.mlp.h1, sm.mlp.h2, sm.mlp.y);
             pred = sm.mlp.y;
         }
-        int32_t htag = 0, used = 0;
-        const int32_t mode = xbfs_choose_mode(a.policy, a.sched, a.sched_len, pred, &st.hs, L, m_f,
-                                              a.m, a.alpha, &htag, &used);
+        if (threadIdx.x == 0) {
+            int32_t htag = 0, used = 0;
+            ls.mode = xbfs_choose_mode(a.policy, a.sched, a.sched_len, pred, &ls.st.hs, L, ls.m_f,
+                                       a.m, a.alpha, &htag, &used);
+            ls.htag = htag;
+            ls.used = used;
+        }
+        __syncthreads();
+        const int32_t mode = ls.mode;
         ph_level_prep(a, L, bid, nb);
-        if (mode != XBFS_MODE_BU && !st.q_valid && !st.need_sweep) {
+        if (mode != XBFS_MODE_BU && !ls.st.q_valid && !need_sweep) {
             ph_sweep(a, L, bid, nb, false, false, lds);
             if (!xb_grid_sync(c)) return;
         }
-        if (mode == XBFS_MODE_TD_SF) ph_td<true>(a, L, bid, nb, p2_next == 0, sm);
+        if (mode == XBFS_MODE_TD_SF) ph_td<true>(a, L, bid, nb, ls.p2_next == 0, sm);
         else if (mode == XBFS_MODE_TD_SS) ph_td<false>(a, L, bid, nb, false, sm);
         else ph_bu(a, L, bid, nb, lds);
         if (!xb_grid_sync(c)) return;
-        if (bid == 0 && threadIdx.x == 0 && L < a.rec_cap) {
-            XbDevRec &r = a.recs[L];
-            r.level = L;
-            r.mode = mode;
-            r.tag = xbfs_mode_to_tag(mode);
-            r.htag = (htag & 0xff) | (used << 8);
-            r.n_f = n_f;
-            r.m_f = m_f;
-            r.n_u = n_u;
-            r.m_u = m_u;
-            r.t0 = (long long)t0;
-            r.t1 = (long long)wall_clock64();
+        if (threadIdx.x == 0) {
+            if (bid == 0 && L < a.rec_cap) {
+                XbDevRec &r = a.recs[L];
+                r.level = L;
+                r.mode = mode;
+                r.tag = xbfs_mode_to_tag(mode);
+                r.htag = (ls.htag & 0xff) | (ls.used << 8);
+                r.n_f = ls.n_f;
+                r.m_f = ls.m_f;
+                r.n_u = ls.n_u;
+                r.m_u = ls.m_u;
+                r.t0 = (long long)t0;
+                r.t1 = (long long)wall_clock64();
+            }
+            ls.st.q_valid = (mode == XBFS_MODE_TD_SF && ls.p2_next == 0);
+            ls.st.need_sweep = (mode == XBFS_MODE_TD_SS);
+            ls.st.prev_mode = mode;
+            ls.st.L = L + 1;
         }
-        st.q_valid = (mode == XBFS_MODE_TD_SF && p2_next == 0);
-        st.need_sweep = (mode == XBFS_MODE_TD_SS);
-        st.prev_mode = mode;
-        st.L = L + 1;
+        __syncthreads();
     }
 }
 
@@ -842,6 +946,7 @@ __global__ void __launch_bounds__(XB_PBLOCK) k_ph_bu(XbArgs a, int32_t L)
     extern __shared__ __align__(16) unsigned char xb_dyn_lds[];
     ph_bu(a, L, blockIdx.x, gridDim.x, reinterpret_cast<XbWaveLds *>(xb_dyn_lds));
 }
+
 
 // Host-driven loop: sums every field of the partial statistics into
 // ctl->sum (one block).
