@@ -101,7 +101,7 @@ union XbSmem {
         int32_t u[XB_GCAP];
     } td;
     struct {
-        float x[8], h1[32], h2[32], y[4];
+        float x[8], y[4];
     } mlp;
 };
 

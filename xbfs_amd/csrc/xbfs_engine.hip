@@ -75,8 +75,9 @@ __device__ void xb_part_add(const XbArgs &a, int bid, unsigned long long (&v)[K]
     __syncthreads();
 }
 
-// Block-cooperative sums of K fields over all XB_PMAX columns; every thread
-// of the block gets the results.
+// Block-cooperative sums of K fields over the columns of the grid's blocks
+// (columns >= gridDim.x are never written in a launch and stay zero from
+// ph_init); every thread of the block gets the results.
 template <int K>
 __device__ void xb_part_sum(const XbArgs &a, const int (&f)[K], long long (&out)[K])
 {
@@ -86,7 +87,7 @@ __device__ void xb_part_sum(const XbArgs &a, const int (&f)[K], long long (&out)
 #pragma unroll
     for (int k = 0; k < K; k++) {
         acc[k] = 0;
-        for (int b = threadIdx.x; b < XB_PMAX; b += blockDim.x)
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
             acc[k] += a.part[(int64_t)f[k] * XB_PMAX + b];
         const long long t = (long long)xb_wave_sum_u64((unsigned long long)acc[k]);
         if (xb_lane() == 0) sh[k][w] = t;
@@ -319,7 +320,7 @@ __device__ void ph_sweep(const XbArgs &a, int32_t L, int bid, int nb, bool from_
     __shared__ int s_base[XB_BINS];
     uint64_t *F = xb_f(a, L);
     const int qb = L % XB_QBUFS;
-    int32_t *const q[XB_BINS] = {xb_q(a, qb, 0), xb_q(a, qb, 1), xb_q(a, qb, 2), xb_q(a, qb, 3)};
+    int32_t *const qd[XB_BINS] = {xb_q(a, qb, 0), xb_q(a, qb, 1), xb_q(a, qb, 2), xb_q(a, qb, 3)};
     XbCounter *gcnt = a.ctl->qcnt[qb];
     const int ln = xb_lane(), wv = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
     XbWaveLds *wl = lds + wv;
@@ -328,8 +329,21 @@ __device__ void ph_sweep(const XbArgs &a, int32_t L, int bid, int nb, bool from_
     const int64_t nchunks = (a.nwords + XB_CHUNK_WORDS - 1) / XB_CHUNK_WORDS;
     unsigned long long my_n = 0, my_m = 0;
     int wc[XB_BINS] = {0, 0, 0, 0};
-    // pass 1: bits (from the levels when asked), statistics, bin counts
-    for (int64_t ch = gw; ch < nchunks; ch += nwv) {
+    // pass 1: bits (from the levels when asked), statistics, bin counts.
+    // Chunks in groups of 4 as in ph_bu: one round trip loads the F_L words
+    // of 4 chunks (lane 16q + k: word k of chunk q); empty chunks are skipped.
+    for (int64_t cg = gw; cg < nchunks; cg += 4 * nwv) {
+    uint64_t pref = 0;
+    if (!from_levels) {
+        const int64_t chl = cg + (int64_t)(ln >> 4) * nwv;
+        const int64_t wq = chl * XB_CHUNK_WORDS + (ln & 15);
+        if (chl < nchunks && wq < a.nwords) pref = F[wq];
+    }
+    const unsigned long long nzm = from_levels ? ~0ull : __ballot(pref != 0ull);
+    for (int q = 0; q < 4; q++) {
+        const int64_t ch = cg + (int64_t)q * nwv;
+        if (ch >= nchunks) break;
+        if (((nzm >> (16 * q)) & 0xffffull) == 0ull) continue;
         const int64_t w0 = ch * XB_CHUNK_WORDS;
         const int64_t w = w0 + ln;
         uint64_t myword = 0;
@@ -351,8 +365,9 @@ __device__ void ph_sweep(const XbArgs &a, int32_t L, int bid, int nb, bool from_
                 F[w] = myword;
                 if (myword) a.vis[w] |= myword;
             }
-        } else if (ln < XB_CHUNK_WORDS && w < a.nwords) {
-            myword = F[w];
+        } else {
+            const uint64_t pq = __shfl(pref, 16 * q + (ln & 15));
+            myword = ln < XB_CHUNK_WORDS ? pq : 0ull;
         }
         const int total = xb_chunk_list(myword, w0, wl);
         for (int r = 0; r < total; r += 64) {
@@ -369,6 +384,7 @@ __device__ void ph_sweep(const XbArgs &a, int32_t L, int bid, int nb, bool from_
             for (int k = 0; k < XB_BINS; k++) wc[k] += (int)__popcll(__ballot(bin == k));
         }
         xb_wave_sync();
+    }
     }
     // reserve: one atomic per bin per block
     if (ln == 0)
@@ -387,12 +403,20 @@ __device__ void ph_sweep(const XbArgs &a, int32_t L, int bid, int nb, bool from_
     __syncthreads();
     int pos[XB_BINS];
     for (int k = 0; k < XB_BINS; k++) pos[k] = s_base[k] + s_wc[wv][k];
-    // pass 2: write the queue entries
-    for (int64_t ch = gw; ch < nchunks; ch += nwv) {
-        const int64_t w0 = ch * XB_CHUNK_WORDS;
-        const int64_t w = w0 + ln;
-        uint64_t myword = 0;
-        if (ln < XB_CHUNK_WORDS && w < a.nwords) myword = F[w];
+    // pass 2: write the queue entries (same chunk groups)
+    for (int64_t cg = gw; cg < nchunks; cg += 4 * nwv) {
+    uint64_t pref = 0;
+    {
+        const int64_t chl = cg + (int64_t)(ln >> 4) * nwv;
+        const int64_t wq = chl * XB_CHUNK_WORDS + (ln & 15);
+        if (chl < nchunks && wq < a.nwords) pref = F[wq];
+    }
+    const unsigned long long nzm = __ballot(pref != 0ull);
+    for (int q = 0; q < 4; q++) {
+        if (((nzm >> (16 * q)) & 0xffffull) == 0ull) continue;
+        const int64_t w0 = (cg + (int64_t)q * nwv) * XB_CHUNK_WORDS;
+        const uint64_t pq = __shfl(pref, 16 * q + (ln & 15));
+        const uint64_t myword = ln < XB_CHUNK_WORDS ? pq : 0ull;
         const int total = xb_chunk_list(myword, w0, wl);
         for (int r = 0; r < total; r += 64) {
             const int i = r + ln;
@@ -406,11 +430,12 @@ __device__ void ph_sweep(const XbArgs &a, int32_t L, int bid, int nb, bool from_
 #pragma unroll
             for (int k = 0; k < XB_BINS; k++) {
                 const unsigned long long m = __ballot(bin == k);
-                if (bin == k) q[k][pos[k] + xb_rank(m)] = v;
+                if (bin == k) qd[k][pos[k] + xb_rank(m)] = v;
                 pos[k] += (int)__popcll(m);
             }
         }
         xb_wave_sync();
+    }
     }
     if (stats) {
         unsigned long long vv[2] = {my_n, my_m};
@@ -571,11 +596,25 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
     const int64_t nchunks = (a.nwords + XB_CHUNK_WORDS - 1) / XB_CHUNK_WORDS;
     const uint64_t below = ln ? (~0ull >> (64 - ln)) : 0ull;
     unsigned long long n1 = 0, m1 = 0, n2 = 0, m2 = 0;
-    for (int64_t ch = gw; ch < nchunks; ch += nwv) {
+    // The wave's chunks gw, gw + W, gw + 2W, ... in groups of 4: lane 16q + k
+    // loads word k of the group's chunk q, so one memory round trip covers 4
+    // chunks and empty chunks are skipped by one ballot.
+    for (int64_t cg = gw; cg < nchunks; cg += 4 * nwv) {
+    uint64_t pref = 0;
+    {
+        const int64_t chl = cg + (int64_t)(ln >> 4) * nwv;
+        const int64_t wq = chl * XB_CHUNK_WORDS + (ln & 15);
+        if (chl < nchunks && wq < a.nwords) pref = a.nz[wq] & ~a.vis[wq];
+    }
+    const unsigned long long nzm = __ballot(pref != 0ull);
+    for (int q = 0; q < 4; q++) {
+        if (((nzm >> (16 * q)) & 0xffffull) == 0ull) continue;
+        const int64_t ch = cg + (int64_t)q * nwv;
         const int64_t w0 = ch * XB_CHUNK_WORDS;
         const int64_t w = w0 + ln;
-        uint64_t cand = 0; // lane k < 16: unvisited non-isolated vertices of word w0 + k
-        if (ln < XB_CHUNK_WORDS && w < a.nwords) cand = a.nz[w] & ~a.vis[w];
+        // lane k < 16: unvisited non-isolated vertices of word w0 + k
+        const uint64_t pq = __shfl(pref, 16 * q + (ln & 15));
+        const uint64_t cand = ln < XB_CHUNK_WORDS ? pq : 0ull;
         const int pc = (int)__popcll(cand);
         const int inc = xb_wave_incl_scan_i32(pc);
         const int total = __shfl(inc, 63);
@@ -713,6 +752,7 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
             }
         }
     }
+    }
     // n2 also goes to the +2 field: a top-down level L+1 must not trust its winner
     // queue as the level-(L+2) frontier when +2 discoveries exist
     unsigned long long vv[5] = {n1, m1, n2, m2, n2};
@@ -739,33 +779,34 @@ __host__ __device__ inline bool xb_level_open(XbLoop &st, const XbArgs &a, long 
     return true;
 }
 
-// Block-parallel MLP forward (thread o < 32 computes hidden unit o); all
-// threads of the block must call it.  Writes the three predictions to y.
-__device__ void xb_mlp_block_forward(const float *p, const float *x, float *s_h1, float *s_h2,
-                                     float *y)
+// MLP forward by one wave from LDS weights p (packed as in xbfs_policy.h):
+// lane o < 32 computes hidden unit o, the hidden vectors move between lanes
+// by shuffles, lanes 0..2 write the outputs to y.  Sums run in input order,
+// as in xbfs_mlp_forward1.  Called by every lane of the wave.
+__device__ void xb_mlp_wave_forward(const float *p, const float *x, float *y)
 {
     const float *W1 = p, *b1 = W1 + XB_MLP_NH * XB_MLP_NI;
     const float *W2 = b1 + XB_MLP_NH, *b2 = W2 + XB_MLP_NH * XB_MLP_NH;
     const float *W3 = b2 + XB_MLP_NH, *b3 = W3 + XB_MLP_NO * XB_MLP_NH;
-    const int o = threadIdx.x;
+    const int o = xb_lane();
+    float h1 = 0.0f;
     if (o < XB_MLP_NH) {
-        float s = b1[o];
-        for (int i = 0; i < XB_MLP_NI; i++) s += W1[o * XB_MLP_NI + i] * x[i];
-        s_h1[o] = s > 0.0f ? s : 0.0f;
+        float t = b1[o];
+        for (int i = 0; i < XB_MLP_NI; i++) t += W1[o * XB_MLP_NI + i] * x[i];
+        h1 = t > 0.0f ? t : 0.0f;
     }
-    __syncthreads();
-    if (o < XB_MLP_NH) {
-        float s = b2[o];
-        for (int i = 0; i < XB_MLP_NH; i++) s += W2[o * XB_MLP_NH + i] * s_h1[i];
-        s_h2[o] = s > 0.0f ? s : 0.0f;
+    float t2 = o < XB_MLP_NH ? b2[o] : 0.0f;
+    for (int i = 0; i < XB_MLP_NH; i++) {
+        const float hi = __shfl(h1, i);
+        if (o < XB_MLP_NH) t2 += W2[o * XB_MLP_NH + i] * hi;
     }
-    __syncthreads();
-    if (o < XB_MLP_NO) {
-        float s = b3[o];
-        for (int i = 0; i < XB_MLP_NH; i++) s += W3[o * XB_MLP_NH + i] * s_h2[i];
-        y[o] = s;
+    const float h2 = (o < XB_MLP_NH && t2 > 0.0f) ? t2 : 0.0f;
+    float t3 = o < XB_MLP_NO ? b3[o] : 0.0f;
+    for (int i = 0; i < XB_MLP_NH; i++) {
+        const float hi = __shfl(h2, i);
+        if (o < XB_MLP_NO) t3 += W3[o * XB_MLP_NH + i] * hi;
     }
-    __syncthreads();
+    if (o < XB_MLP_NO) y[o] = t3;
 }
 
 // The level loop's block-uniform state lives in LDS (thread 0 updates it,
@@ -782,8 +823,11 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
     const int bid = blockIdx.x, nb = gridDim.x;
     __shared__ XbSmem sm;
     __shared__ XbLoopShared ls;
+    __shared__ float s_mlp[XB_MLP_NPARAM]; // learned policy weights, loaded once
     extern __shared__ __align__(16) unsigned char xb_dyn_lds[];
     XbWaveLds *lds = reinterpret_cast<XbWaveLds *>(xb_dyn_lds);
+    if (a.mlp)
+        for (int i = threadIdx.x; i < XB_MLP_NPARAM; i += blockDim.x) s_mlp[i] = a.mlp[i];
     ph_init(a, bid, nb);
     if (!xb_grid_sync(a.ctl)) return;
     if (a.nsrc > 0) {
@@ -834,20 +878,24 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
             __syncthreads();
         }
         if (!ls.open) return;
-        const float *pred = nullptr;
-        if (a.mlp) {
-            if (threadIdx.x == 0)
-                xbfs_mlp_features(L, ls.n_f, ls.m_f, ls.n_u, ls.m_u, ls.st.prev_mode, sm.mlp.x);
-            __syncthreads();
-            xb_mlp_block_forward(a.mlp, sm.mlp.x, sm.mlp.h1, sm.mlp.h2, sm.mlp.y);
-            pred = sm.mlp.y;
-        }
-        if (threadIdx.x == 0) {
-            int32_t htag = 0, used = 0;
-            ls.mode = xbfs_choose_mode(a.policy, a.sched, a.sched_len, pred, &ls.st.hs, L, ls.m_f,
-                                       a.m, a.alpha, &htag, &used);
-            ls.htag = htag;
-            ls.used = used;
+        // the decision: wave 0 alone (features, MLP forward, chooser)
+        if (threadIdx.x < 64) {
+            const float *pred = nullptr;
+            if (a.mlp) {
+                if (threadIdx.x == 0)
+                    xbfs_mlp_features(L, ls.n_f, ls.m_f, ls.n_u, ls.m_u, ls.st.prev_mode, sm.mlp.x);
+                xb_wave_sync();
+                xb_mlp_wave_forward(s_mlp, sm.mlp.x, sm.mlp.y);
+                xb_wave_sync();
+                pred = sm.mlp.y;
+            }
+            if (threadIdx.x == 0) {
+                int32_t htag = 0, used = 0;
+                ls.mode = xbfs_choose_mode(a.policy, a.sched, a.sched_len, pred, &ls.st.hs, L,
+                                           ls.m_f, a.m, a.alpha, &htag, &used);
+                ls.htag = htag;
+                ls.used = used;
+            }
         }
         __syncthreads();
         const int32_t mode = ls.mode;
