@@ -48,7 +48,7 @@ bool pow2_block(int b) { return b >= 64 && b <= 1024 && (b & (b - 1)) == 0; }
 int xb_ctx_free_device(xbfs_ctx *c)
 {
     (void)hipSetDevice(c->device);
-    void *bufs[] = {c->d_beg,   c->d_adj,  c->d_nz,    c->d_vis,  c->d_fbm,   c->d_level,
+    void *bufs[] = {c->d_beg,   c->d_adj,  c->d_nz,    c->d_hd,   c->d_vis,  c->d_fbm,   c->d_level,
                     c->d_parent, c->d_qbuf, c->d_ctl,  c->d_recs, c->d_sched, c->d_mlp,
                     c->d_list,  c->d_ubits, c->d_tile, c->d_list_cnt, c->s_level,
                     c->s_vis,   c->s_fbm,   c->s_qbuf, c->s_ctl, c->d_part, c->s_part,
@@ -139,6 +139,7 @@ int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, i
     bool ok = hipMalloc(&c->d_beg, sizeof(int64_t) * (size_t)(n + 1)) == hipSuccess &&
               hipMalloc(&c->d_adj, sizeof(int32_t) * mm) == hipSuccess &&
               hipMalloc(&c->d_nz, sizeof(uint64_t) * nw) == hipSuccess &&
+              hipMalloc(&c->d_hd, sizeof(int2) * (size_t)(n > 0 ? n : 1)) == hipSuccess &&
               hipMalloc(&c->d_vis, sizeof(uint64_t) * nw) == hipSuccess &&
               hipMalloc(&c->d_fbm, sizeof(uint64_t) * (size_t)c->fstride * XB_FRINGS) == hipSuccess &&
               hipMalloc(&c->d_level, sizeof(int32_t) * (size_t)n) == hipSuccess &&
@@ -165,6 +166,12 @@ int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, i
         return fail(XBFS_ERR_HIP);
     hipLaunchKernelGGL(k_build_nz, dim3(grid_for(c->nwords * 64, 256, 65536)), dim3(256), 0,
                        c->stream, c->d_beg, n, c->d_nz, c->nwords);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_build_hd, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream,
+                           c->d_beg, c->d_adj, n, c->d_hd);
+        hipLaunchKernelGGL(k_build_hd_big, dim3(c->num_cu * 8), dim3(256), 0, c->stream, c->d_beg,
+                           c->d_adj, n, c->d_hd);
+    }
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
         return fail(XBFS_ERR_HIP);
     *out = c;
@@ -222,6 +229,7 @@ static XbArgs make_args(xbfs_ctx *c, int32_t src, const xbfs_opts &o)
     a.beg = c->d_beg;
     a.adj = c->d_adj;
     a.nz = c->d_nz;
+    a.hd = c->d_hd;
     a.vis = c->d_vis;
     a.fbm = c->d_fbm;
     a.qbuf = c->d_qbuf;

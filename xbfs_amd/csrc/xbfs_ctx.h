@@ -25,6 +25,7 @@ struct xbfs_ctx {
     int64_t *d_beg = nullptr;   // int64[n+1]
     int32_t *d_adj = nullptr;   // int32[m]
     uint64_t *d_nz = nullptr;   // bit v: deg(v) > 0
+    int2 *d_hd = nullptr;       // {largest-degree neighbour, degree} per vertex
     uint64_t *d_vis = nullptr;  // visited bitmap
     uint64_t *d_fbm = nullptr;  // 4 frontier bitmaps (ring by level & 3)
     int32_t *d_level = nullptr; // int32[n]

@@ -65,6 +65,7 @@ struct XbArgs {
     const int64_t *beg;
     const int32_t *adj;
     const uint64_t *nz; // bit v: deg(v) > 0
+    const int2 *hd;     // {largest-degree neighbour, degree} per vertex
     uint64_t *vis;      // bit v: level[v] != -1
     uint64_t *fbm;      // XB_FRINGS bitmaps of nwords: bit v of ring l&3 <=> level[v] == l
     int32_t *qbuf;      // XB_QBUFS x XB_BINS x n

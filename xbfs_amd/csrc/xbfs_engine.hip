@@ -633,71 +633,127 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
             wl->va[ln] = 0ull;
         }
         xb_wave_sync();
-        int ndef = 0; // deferred candidates, stored at the front of the list
-        for (int r = 0; r < total; r += 64) {
-            const int i = r + ln;
+        // Hub probe, 4 candidates per lane per round (independent loads): a
+        // candidate whose largest-degree neighbour is in F_L gets level L+1
+        // with that neighbour as parent, without reading its adjacency.  The
+        // others are compacted to the front of the list (positions below the
+        // entries read so far) for the scan below.
+        int nres = 0;
+        for (int r = 0; r < total; r += 256) {
+            int32_t vc[4];
+            int2 h[4];
+            bool hit[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int i = r + 64 * t + ln;
+                vc[t] = i < total ? wl->list[i] : -1;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) h[t] = vc[t] >= 0 ? a.hd[vc[t]] : make_int2(-1, 0);
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                hit[t] = h[t].x >= 0 && ((F[h[t].x >> 6] >> (h[t].x & 63)) & 1ull);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                if (hit[t]) {
+                    const int32_t v = vc[t];
+                    const unsigned long long bit = 1ull << (v & 63);
+                    const int li = (int)((v >> 6) - w0);
+                    a.level[v] = L1;
+                    if (a.parent) a.parent[v] = h[t].x;
+                    atomicOr(&wl->f1[li], bit);
+                    atomicOr(&wl->va[li], bit);
+                    n1 += 1;
+                    m1 += (unsigned long long)h[t].y;
+                }
+            }
+            xb_wave_sync();
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const bool un = vc[t] >= 0 && !hit[t];
+                const unsigned long long um = __ballot(un);
+                if (un) wl->list[nres + xb_rank(um)] = vc[t];
+                nres += (int)__popcll(um);
+            }
+            xb_wave_sync();
+        }
+        // Each lane works on its own candidate, 4 neighbours per step; a lane
+        // whose candidate is settled takes the next one from the list, so a
+        // long scan in one lane does not hold the others.  At most
+        // XB_BU_LANE_MAX edges per lane: a candidate still unresolved with
+        // edges left is deferred to the front of the list (positions < the
+        // consumed count, so no unread entry is overwritten) and finished by
+        // the whole wave below.
+        int ndef = 0, next = 0;
+        int32_t v = -1, seen = -1;
+        int64_t j = 0, ce = 0, e = 0, b = 0;
+        for (;;) {
+            const bool idle = v < 0;
+            const unsigned long long im = __ballot(idle);
+            if (idle) {
+                const int k = next + xb_rank(im);
+                if (k < nres) {
+                    v = wl->list[k];
+                    b = a.beg[v];
+                    e = a.beg[v + 1];
+                    j = b;
+                    ce = (e - b > XB_BU_LANE_MAX) ? b + XB_BU_LANE_MAX : e;
+                    seen = -1;
+                }
+            }
+            next = min(nres, next + (int)__popcll(im));
+            if (__ballot(v >= 0) == 0ull) break;
             bool defer = false;
-            int32_t v = 0;
-            if (i < total) {
-                v = wl->list[i];
-                const int64_t b = a.beg[v], e = a.beg[v + 1];
-                // at most XB_BU_LANE_MAX edges per lane (4 neighbours per step,
-                // independent loads); longer scans are finished by the wave below
-                const int64_t ce = (e - b > XB_BU_LANE_MAX) ? b + XB_BU_LANE_MAX : e;
-                int32_t found = -1, seen = -1;
-                int64_t j = b;
-                for (; j + 4 <= ce; j += 4) {
-                    const int32_t u0 = a.adj[j], u1 = a.adj[j + 1], u2 = a.adj[j + 2],
-                                  u3 = a.adj[j + 3];
-                    const bool h0 = (F[u0 >> 6] >> (u0 & 63)) & 1ull;
-                    const bool h1 = (F[u1 >> 6] >> (u1 & 63)) & 1ull;
-                    const bool h2 = (F[u2 >> 6] >> (u2 & 63)) & 1ull;
-                    const bool h3 = (F[u3 >> 6] >> (u3 & 63)) & 1ull;
-                    if (h0 | h1 | h2 | h3) {
-                        found = h0 ? u0 : (h1 ? u1 : (h2 ? u2 : u3));
-                        break;
-                    }
-                    if (seen < 0) {
-                        if ((F1[u0 >> 6] >> (u0 & 63)) & 1ull) seen = u0;
-                        else if ((F1[u1 >> 6] >> (u1 & 63)) & 1ull) seen = u1;
-                        else if ((F1[u2 >> 6] >> (u2 & 63)) & 1ull) seen = u2;
-                        else if ((F1[u3 >> 6] >> (u3 & 63)) & 1ull) seen = u3;
-                    }
-                }
-                if (found < 0) {
-                    for (; j < ce; j++) {
-                        const int32_t u = a.adj[j];
-                        if ((F[u >> 6] >> (u & 63)) & 1ull) { found = u; break; }
-                        if (seen < 0 && ((F1[u >> 6] >> (u & 63)) & 1ull)) seen = u;
-                    }
-                }
+            const int32_t vv = v;
+            if (v >= 0) {
+                const int32_t u0 = a.adj[j];
+                const int32_t u1 = j + 1 < ce ? a.adj[j + 1] : -1;
+                const int32_t u2 = j + 2 < ce ? a.adj[j + 2] : -1;
+                const int32_t u3 = j + 3 < ce ? a.adj[j + 3] : -1;
+                const bool h0 = (F[u0 >> 6] >> (u0 & 63)) & 1ull;
+                const bool h1 = u1 >= 0 && ((F[u1 >> 6] >> (u1 & 63)) & 1ull);
+                const bool h2 = u2 >= 0 && ((F[u2 >> 6] >> (u2 & 63)) & 1ull);
+                const bool h3 = u3 >= 0 && ((F[u3 >> 6] >> (u3 & 63)) & 1ull);
                 const unsigned long long bit = 1ull << (v & 63);
                 const int li = (int)((v >> 6) - w0);
-                if (found >= 0) {
+                if (h0 | h1 | h2 | h3) {
                     a.level[v] = L1;
-                    if (a.parent) a.parent[v] = found;
+                    if (a.parent) a.parent[v] = h0 ? u0 : (h1 ? u1 : (h2 ? u2 : u3));
                     atomicOr(&wl->f1[li], bit);
                     atomicOr(&wl->va[li], bit);
                     n1 += 1;
                     m1 += (unsigned long long)(e - b);
-                } else if (ce < e) {
-                    defer = true; // unscanned edges remain: no +2 decision yet
-                } else if (seen >= 0) {
-                    a.level[v] = L2;
-                    if (a.parent) a.parent[v] = seen;
-                    atomicOr(&wl->f2[li], bit);
-                    atomicOr(&wl->va[li], bit);
-                    n2 += 1;
-                    m2 += (unsigned long long)(e - b);
+                    v = -1;
+                } else {
+                    if (seen < 0) {
+                        if ((F1[u0 >> 6] >> (u0 & 63)) & 1ull) seen = u0;
+                        else if (u1 >= 0 && ((F1[u1 >> 6] >> (u1 & 63)) & 1ull)) seen = u1;
+                        else if (u2 >= 0 && ((F1[u2 >> 6] >> (u2 & 63)) & 1ull)) seen = u2;
+                        else if (u3 >= 0 && ((F1[u3 >> 6] >> (u3 & 63)) & 1ull)) seen = u3;
+                    }
+                    j += 4;
+                    if (j >= ce) {
+                        if (ce < e) {
+                            defer = true; // unscanned edges remain: no +2 decision yet
+                        } else if (seen >= 0) {
+                            a.level[v] = L2;
+                            if (a.parent) a.parent[v] = seen;
+                            atomicOr(&wl->f2[li], bit);
+                            atomicOr(&wl->va[li], bit);
+                            n2 += 1;
+                            m2 += (unsigned long long)(e - b);
+                        }
+                        v = -1;
+                    }
                 }
             }
-            // deferred vertices go to list[0, ndef): every lane has read its
-            // entry of this round, and later rounds read at indices >= r + 64
             const unsigned long long dm = __ballot(defer);
-            xb_wave_sync();
-            if (defer) wl->list[ndef + xb_rank(dm)] = v;
-            ndef += (int)__popcll(dm);
-            xb_wave_sync();
+            if (dm) {
+                xb_wave_sync();
+                if (defer) wl->list[ndef + xb_rank(dm)] = vv;
+                ndef += (int)__popcll(dm);
+                xb_wave_sync();
+            }
         }
         // The deferred candidates, one at a time by the whole wave: 256 edges
         // per round (4 coalesced adjacency words per lane), a ballot ends the

@@ -39,6 +39,77 @@ __global__ void __launch_bounds__(256) k_build_nz(const int64_t *__restrict__ be
     }
 }
 
+// Hub table: hd[v] = {the neighbour of v with the largest degree (first in
+// adjacency order on ties), deg(v)}; {-1, 0} for an isolated vertex.
+// Bottom-up probes the frontier bit of that neighbour before touching v's
+// adjacency.  Lane per vertex for deg <= XB_HD_LANE_MAX; larger vertices
+// are marked {-2, deg} and finished by k_build_hd_big.
+#define XB_HD_LANE_MAX 256
+__global__ void __launch_bounds__(256) k_build_hd(const int64_t *__restrict__ beg,
+                                                  const int32_t *__restrict__ adj, int64_t n,
+                                                  int2 *__restrict__ hd)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const int64_t b = beg[v], e = beg[v + 1];
+        const int32_t d = (int32_t)(e - b);
+        int32_t best = -1;
+        if (d > XB_HD_LANE_MAX) {
+            best = -2;
+        } else {
+            int64_t bd = -1;
+            for (int64_t j = b; j < e; j++) {
+                const int32_t u = adj[j];
+                const int64_t du = beg[u + 1] - beg[u];
+                if (du > bd) {
+                    bd = du;
+                    best = u;
+                }
+            }
+        }
+        hd[v] = make_int2(best, d);
+    }
+}
+
+// Marked vertices, one wave each: a wave checks 64 vertices per load, then
+// for each marked one its lanes stride the adjacency and a wave reduction on
+// (degree, -position) keeps the first neighbour of largest degree.
+__global__ void __launch_bounds__(256) k_build_hd_big(const int64_t *__restrict__ beg,
+                                                      const int32_t *__restrict__ adj, int64_t n,
+                                                      int2 *__restrict__ hd)
+{
+    const int64_t nwv = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int ln = xb_lane();
+    for (int64_t v0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64; v0 < n;
+         v0 += nwv * 64) {
+        const bool mk = v0 + ln < n && hd[v0 + ln].x == -2;
+        unsigned long long m = __ballot(mk);
+        while (m) {
+            const int k = __ffsll(m) - 1;
+            m &= m - 1;
+            const int64_t v = v0 + k;
+            const int64_t b = beg[v], e = beg[v + 1];
+            long long bd = -1, bp = 0x7fffffffffffffffll;
+            for (int64_t j = b + ln; j < e; j += 64) {
+                const int32_t u = adj[j];
+                const long long du = beg[u + 1] - beg[u];
+                if (du > bd) { // first (smallest j) of this lane's maximum
+                    bd = du;
+                    bp = j;
+                }
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const long long od = __shfl_xor(bd, o), op = __shfl_xor(bp, o);
+                if (od > bd || (od == bd && op < bp)) {
+                    bd = od;
+                    bp = op;
+                }
+            }
+            if (ln == 0) hd[v] = make_int2(adj[bp], (int32_t)(e - b));
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // bottom-up: bitmaps + ordered unvisited list
 // ---------------------------------------------------------------------------
