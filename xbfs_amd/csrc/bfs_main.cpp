@@ -18,9 +18,10 @@
 //   * Times are device seconds from HIP events, printed as seconds.
 // Optional environment knobs (the 11-argument form is unchanged):
 //   XBFS_POLICY      heuristic|learned|td_sf|td_ss|bu       (default heuristic)
-//   XBFS_PERSISTENT  1 = device-side level loop             (default 0)
+//   XBFS_PERSISTENT  1 = device-side level loop (default), 0 = host-driven
 //   XBFS_LEVELS_OUT  path: write the int32 level array (raw, n entries)
 //   XBFS_POLICY_WEIGHTS path: float32 MLP weights for the learned policy
+//                    (default: data/policy_mlp.f32 beside the executable)
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -30,8 +31,36 @@
 #include <string>
 #include <vector>
 #include <chrono>
+#include <unistd.h>
 
 #include "xbfs.h"
+
+// Learned-policy weights: XBFS_POLICY_WEIGHTS, else data/policy_mlp.f32 in
+// the executable's directory.  Returns false when none can be read (the
+// library then decides with the heuristic).
+static bool load_weights(xbfs_ctx *ctx)
+{
+    std::string path;
+    if (const char *p = getenv("XBFS_POLICY_WEIGHTS")) {
+        path = p;
+    } else {
+        char buf[4096];
+        const ssize_t k = readlink("/proc/self/exe", buf, sizeof(buf) - 1);
+        if (k <= 0) return false;
+        buf[k] = 0;
+        path = buf;
+        const size_t slash = path.rfind('/');
+        path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) +
+               "/data/policy_mlp.f32";
+    }
+    std::vector<float> w(XBFS_MLP_NPARAM);
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    const size_t got = fread(w.data(), sizeof(float), w.size(), f);
+    fclose(f);
+    if (got != w.size()) return false;
+    return xbfs_policy_load(ctx, w.data(), (int32_t)w.size()) == XBFS_OK;
+}
 
 static int64_t validate(const xbfs_host_graph &g, int32_t src, const std::vector<int32_t> &lv)
 {
@@ -92,11 +121,6 @@ int main(int argc, char **argv)
     std::cout << "Vertex classification based on the degree: \n";
     std::cout << "small degree counts: " << di.small << " medium degree counts: " << di.medium
               << " large degree counts: " << di.large;
-    if (src < 0 || src >= g.n) {
-        std::cout << "\nsource " << src << " out of range [0, " << g.n << ")\n";
-        xbfs_graph_free(&g);
-        return 1;
-    }
     std::cout << "\nStarting BFS graph traversal from node " << src << "......\n";
     std::cout << "\nstarting processing the status array on GPU....\n";
     std::cout << "The average degree of the graph is: " << g.m / g.n;
@@ -107,6 +131,11 @@ int main(int argc, char **argv)
     std::cout << "maximum degree of graph is " << di.max_degree << "\n";
     std::cout << "hub_num = " << di.hub_num << "\n";
 
+    if (src < 0 || src >= g.n) {
+        std::cout << "source " << src << " out of range [0, " << g.n << ")\n";
+        xbfs_graph_free(&g);
+        return 1;
+    }
     xbfs_ctx *ctx = nullptr;
     rc = xbfs_create(gpu, g.beg_pos, g.adj, g.n, g.m, &ctx);
     if (rc != XBFS_OK) {
@@ -123,7 +152,10 @@ int main(int argc, char **argv)
     o.time_levels = 1;
     if (const char *p = getenv("XBFS_POLICY")) {
         std::string s(p);
-        if (s == "learned") o.policy = XBFS_POLICY_LEARNED;
+        if (s == "learned") {
+            o.policy = XBFS_POLICY_LEARNED;
+            if (!load_weights(ctx)) std::cout << "learned policy: no weights, heuristic used\n";
+        }
         else if (s == "td_sf") o.policy = XBFS_POLICY_FORCE_TD_SF;
         else if (s == "td_ss") o.policy = XBFS_POLICY_FORCE_TD_SS;
         else if (s == "bu") o.policy = XBFS_POLICY_FORCE_BU;
