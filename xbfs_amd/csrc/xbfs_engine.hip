@@ -75,31 +75,22 @@ __device__ void xb_part_add(const XbArgs &a, int bid, unsigned long long (&v)[K]
     __syncthreads();
 }
 
-// Block-cooperative sums of K fields over the columns of the grid's blocks
-// (columns >= gridDim.x are never written in a launch and stay zero from
-// ph_init); every thread of the block gets the results.
+// Sums of K fields over the columns of the grid's blocks (columns >=
+// gridDim.x are never written in a launch and stay zero from ph_init),
+// computed by wave 0 alone: every lane issues all its loads at once, then
+// one wave reduction.  Lane 0 of wave 0 gets the results; other threads must
+// not use out.
 template <int K>
 __device__ void xb_part_sum(const XbArgs &a, const int (&f)[K], long long (&out)[K])
 {
-    __shared__ long long sh[K][16];
-    const int w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
-    long long acc[K];
+    if (threadIdx.x >= 64) return;
+    const int ln = xb_lane(), nb = (int)gridDim.x;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        acc[k] = 0;
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-            acc[k] += a.part[(int64_t)f[k] * XB_PMAX + b];
-        const long long t = (long long)xb_wave_sum_u64((unsigned long long)acc[k]);
-        if (xb_lane() == 0) sh[k][w] = t;
+        long long acc = 0;
+        for (int b = ln; b < nb; b += 64) acc += a.part[(int64_t)f[k] * XB_PMAX + b];
+        out[k] = (long long)xb_wave_sum_u64((unsigned long long)acc);
     }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        long long t = 0;
-        for (int i = 0; i < nw; i++) t += sh[k][i];
-        out[k] = t;
-    }
-    __syncthreads();
 }
 
 // Grid-wide barrier for a cooperative launch (all blocks co-resident).  The
@@ -845,23 +836,19 @@ __device__ void xb_mlp_wave_forward(const float *p, const float *x, float *y)
     const float *W2 = b1 + XB_MLP_NH, *b2 = W2 + XB_MLP_NH * XB_MLP_NH;
     const float *W3 = b2 + XB_MLP_NH, *b3 = W3 + XB_MLP_NO * XB_MLP_NH;
     const int o = xb_lane();
-    float h1 = 0.0f;
-    if (o < XB_MLP_NH) {
-        float t = b1[o];
-        for (int i = 0; i < XB_MLP_NI; i++) t += W1[o * XB_MLP_NI + i] * x[i];
-        h1 = t > 0.0f ? t : 0.0f;
-    }
-    float t2 = o < XB_MLP_NH ? b2[o] : 0.0f;
-    for (int i = 0; i < XB_MLP_NH; i++) {
-        const float hi = __shfl(h1, i);
-        if (o < XB_MLP_NH) t2 += W2[o * XB_MLP_NH + i] * hi;
-    }
+    const int oh = o < XB_MLP_NH ? o : 0, oo = o < XB_MLP_NO ? o : 0;
+    // fully unrolled: every weight load and every lane read is issued up front
+    float t1 = b1[oh];
+#pragma unroll
+    for (int i = 0; i < XB_MLP_NI; i++) t1 += W1[oh * XB_MLP_NI + i] * x[i];
+    const float h1 = (o < XB_MLP_NH && t1 > 0.0f) ? t1 : 0.0f;
+    float t2 = b2[oh];
+#pragma unroll
+    for (int i = 0; i < XB_MLP_NH; i++) t2 += W2[oh * XB_MLP_NH + i] * __shfl(h1, i);
     const float h2 = (o < XB_MLP_NH && t2 > 0.0f) ? t2 : 0.0f;
-    float t3 = o < XB_MLP_NO ? b3[o] : 0.0f;
-    for (int i = 0; i < XB_MLP_NH; i++) {
-        const float hi = __shfl(h2, i);
-        if (o < XB_MLP_NO) t3 += W3[o * XB_MLP_NH + i] * hi;
-    }
+    float t3 = b3[oo];
+#pragma unroll
+    for (int i = 0; i < XB_MLP_NH; i++) t3 += W3[oo * XB_MLP_NH + i] * __shfl(h2, i);
     if (o < XB_MLP_NO) y[o] = t3;
 }
 
@@ -904,7 +891,9 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
             ph_sweep(a, L, bid, nb, true, true, lds);
             if (!xb_grid_sync(c)) return;
         }
-        {
+        // Level statistics and the decision, by wave 0 alone; one block
+        // barrier publishes them.
+        if (threadIdx.x < 64) {
             long long sv[5];
             const int ff[5] = {need_sweep ? xb_f_sweep(L, 0) : xb_f_ring(L, 0),
                                need_sweep ? xb_f_sweep(L, 1) : xb_f_ring(L, 1),
@@ -931,29 +920,29 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
                     ls.open = 0;
                 }
             }
-            __syncthreads();
-        }
-        if (!ls.open) return;
-        // the decision: wave 0 alone (features, MLP forward, chooser)
-        if (threadIdx.x < 64) {
-            const float *pred = nullptr;
-            if (a.mlp) {
-                if (threadIdx.x == 0)
-                    xbfs_mlp_features(L, ls.n_f, ls.m_f, ls.n_u, ls.m_u, ls.st.prev_mode, sm.mlp.x);
-                xb_wave_sync();
-                xb_mlp_wave_forward(s_mlp, sm.mlp.x, sm.mlp.y);
-                xb_wave_sync();
-                pred = sm.mlp.y;
-            }
-            if (threadIdx.x == 0) {
-                int32_t htag = 0, used = 0;
-                ls.mode = xbfs_choose_mode(a.policy, a.sched, a.sched_len, pred, &ls.st.hs, L,
-                                           ls.m_f, a.m, a.alpha, &htag, &used);
-                ls.htag = htag;
-                ls.used = used;
+            xb_wave_sync();
+            if (ls.open) {
+                const float *pred = nullptr;
+                if (a.mlp) {
+                    if (threadIdx.x == 0)
+                        xbfs_mlp_features(L, ls.n_f, ls.m_f, ls.n_u, ls.m_u, ls.st.prev_mode,
+                                          sm.mlp.x);
+                    xb_wave_sync();
+                    xb_mlp_wave_forward(s_mlp, sm.mlp.x, sm.mlp.y);
+                    xb_wave_sync();
+                    pred = sm.mlp.y;
+                }
+                if (threadIdx.x == 0) {
+                    int32_t htag = 0, used = 0;
+                    ls.mode = xbfs_choose_mode(a.policy, a.sched, a.sched_len, pred, &ls.st.hs, L,
+                                               ls.m_f, a.m, a.alpha, &htag, &used);
+                    ls.htag = htag;
+                    ls.used = used;
+                }
             }
         }
         __syncthreads();
+        if (!ls.open) return;
         const int32_t mode = ls.mode;
         ph_level_prep(a, L, bid, nb);
         if (mode != XBFS_MODE_BU && !ls.st.q_valid && !need_sweep) {
