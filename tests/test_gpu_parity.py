@@ -239,3 +239,44 @@ def test_config2_rmat22_full_size():
         if i < 2:
             want, _, _ = O.bfs_queue(g.beg_pos, g.adj, int(s))
             assert (lv == want).all()
+
+
+def test_config3a_er_full_size():
+    """Erdos-Renyi n = 2^24, m = 2^27 endpoint pairs (seed 7), symmetrized:
+    one source bit-exact against the oracle under both drivers, two more
+    Graph500-validated."""
+    g = Graph.er(1 << 24, 1 << 27, seed=7)
+    ctx = Context(g)
+    srcs = seeded_sources(g, 3, seed=7)
+    for i, s in enumerate(srcs):
+        say("er24", s)
+        lv, par, recs, st = ctx.bfs(int(s), want_parent=True, rec_cap=64)
+        assert O.validate(g.beg_pos, g.adj, int(s), lv, par) == 0
+        if i == 0:
+            want, _, _ = O.bfs_queue(g.beg_pos, g.adj, int(s))
+            assert (lv == want).all()
+            lv0, _, _, _ = ctx.bfs(int(s), persistent=0)
+            assert (lv0 == want).all()
+            assert st["v_cc"] == (want >= 0).sum()
+    ctx.close()
+
+
+def test_config3b_grid4096_full_size():
+    """4096 x 4096 grid (16.7M vertices, 8190 levels from a corner): levels
+    equal the Manhattan distance, under both drivers; the persistent kernel
+    runs all 8190 levels in one launch."""
+    R = C = 4096
+    g = Graph.grid(R, C)
+    ctx = Context(g)
+    rr, cc = np.divmod(np.arange(g.n, dtype=np.int64), C)
+    for s, eng in ((0, 1), (R * C - 1, 0), ((R // 2) * C + C // 3, 1)):
+        say("grid4096", s, "persistent", eng)
+        lv, _, _, st = ctx.bfs(s, persistent=eng)
+        sr, sc = divmod(s, C)
+        want = (np.abs(rr - sr) + np.abs(cc - sc)).astype(np.int32)
+        assert (lv == want).all()
+        assert st["max_level"] == want.max()
+        print(f"[grid4096] src {s} persistent {eng}: {st['n_levels']} levels, "
+              f"{st['t_bfs_s'] * 1e3:.1f} ms, {st['t_bfs_s'] / max(st['n_levels'], 1) * 1e6:.1f} "
+              f"us/level", flush=True)
+    ctx.close()
