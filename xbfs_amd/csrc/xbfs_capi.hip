@@ -49,7 +49,7 @@ int xb_ctx_free_device(xbfs_ctx *c)
 {
     (void)hipSetDevice(c->device);
     void *bufs[] = {c->d_beg,   c->d_adj,  c->d_nz,    c->d_hd,   c->d_vis,  c->d_fbm,   c->d_level,
-                    c->d_parent, c->d_qbuf, c->d_ctl,  c->d_recs, c->d_sched, c->d_mlp,
+                    c->d_parent, c->d_qbuf, c->d_ctl,  c->d_lsg, c->d_recs, c->d_sched, c->d_mlp,
                     c->d_list,  c->d_ubits, c->d_tile, c->d_list_cnt, c->s_level,
                     c->s_vis,   c->s_fbm,   c->s_qbuf, c->s_ctl, c->d_part, c->s_part,
                     c->d_srcs,  c->d_b64_deg, c->d_b64_segu, c->d_b64_sege, c->d_b64_F,
@@ -147,6 +147,7 @@ int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, i
               hipMalloc(&c->d_qbuf, sizeof(int32_t) * (size_t)n * XB_QBUFS * XB_BINS) ==
                   hipSuccess &&
               hipMalloc(&c->d_ctl, sizeof(XbCtl)) == hipSuccess &&
+              hipMalloc(&c->d_lsg, sizeof(XbLoopShared)) == hipSuccess &&
               hipMalloc(&c->d_part, sizeof(long long) * XB_NFIELDS * XB_PMAX) == hipSuccess &&
               hipMalloc(&c->d_recs, sizeof(XbDevRec) * XB_REC_CAP) == hipSuccess &&
               hipMalloc(&c->d_sched, XB_REC_CAP) == hipSuccess &&
@@ -234,6 +235,7 @@ static XbArgs make_args(xbfs_ctx *c, int32_t src, const xbfs_opts &o)
     a.fbm = c->d_fbm;
     a.qbuf = c->d_qbuf;
     a.ctl = c->d_ctl;
+    a.lsg = c->d_lsg;
     a.part = c->d_part;
     a.recs = c->d_recs;
     a.rec_cap = XB_REC_CAP;

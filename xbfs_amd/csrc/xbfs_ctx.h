@@ -32,6 +32,7 @@ struct xbfs_ctx {
     int32_t *d_parent = nullptr;
     int32_t *d_qbuf = nullptr;  // 3 queue buffers x 4 degree bins x n
     XbCtl *d_ctl = nullptr;
+    XbLoopShared *d_lsg = nullptr; // persistent kernel: loop state for parked blocks
     long long *d_part = nullptr; // XB_NFIELDS x XB_PMAX partial statistics
     long long *s_part = nullptr; // probe snapshot of d_part
     XbCtl *h_ctl = nullptr;     // pinned mirror

@@ -31,6 +31,12 @@ struct alignas(256) XbCtl {
     uint32_t bar_pad0[31];
     uint32_t bar_gen;                   // grid barrier generation
     uint32_t bar_pad1[31];
+    uint32_t sbar_count;                // small-frontier barrier (active blocks only)
+    uint32_t sbar_pad0[31];
+    uint32_t sbar_gen;
+    uint32_t sbar_pad1[31];
+    uint32_t heartbeat;                 // bumped once per small-frontier level
+    uint32_t hb_pad[31];
     int32_t error;                      // 1 barrier timeout, 2 level cap
     int32_t err_pad[31];
     XbCounter qcnt[XB_QBUFS][XB_BINS];  // queue lengths, one cache line each
@@ -70,6 +76,7 @@ struct XbArgs {
     uint64_t *fbm;      // XB_FRINGS bitmaps of nwords: bit v of ring l&3 <=> level[v] == l
     int32_t *qbuf;      // XB_QBUFS x XB_BINS x n
     XbCtl *ctl;
+    struct XbLoopShared *lsg; // loop state handed to parked blocks (persistent kernel)
     long long *part; // XB_NFIELDS x XB_PMAX partial statistics
     XbDevRec *recs;
     int32_t rec_cap;
@@ -125,6 +132,22 @@ struct XbLoop {
     long long cum_n, cum_m;
     XbfsHeurState hs;
 };
+
+// Block-uniform state of the persistent kernel's level loop (LDS), and its
+// copy in device memory when the parked blocks rejoin.
+struct XbLoopShared {
+    XbLoop st;
+    long long n_f, m_f, n_u, m_u, p2_next;
+    int32_t open, mode, htag, used, small;
+};
+
+// Small-frontier levels: a scan-free top-down level whose queue is exact and
+// whose frontier is at most XB_SMALL_NF vertices and XB_SMALL_MF edges runs
+// on blocks [0, XB_SMALL_BLOCKS) only, with a barrier among those blocks;
+// the other blocks park until a level needs the whole grid.
+#define XB_SMALL_BLOCKS 16
+#define XB_SMALL_NF 16384
+#define XB_SMALL_MF 65536
 
 __host__ __device__ inline void xb_loop_init(XbLoop *s)
 {

@@ -75,28 +75,32 @@ __device__ void xb_part_add(const XbArgs &a, int bid, unsigned long long (&v)[K]
     __syncthreads();
 }
 
-// Sums of K fields over the columns of the grid's blocks (columns >=
-// gridDim.x are never written in a launch and stay zero from ph_init),
-// computed by wave 0 alone: every lane issues all its loads at once, then
-// one wave reduction.  Lane 0 of wave 0 gets the results; other threads must
-// not use out.
+// Sums of K fields over the first ncols columns of the partial statistics
+// (the blocks that wrote them; the zeroing schedule clears every column, so
+// the others are zero), computed by wave 0 alone: every lane issues all its
+// loads at once, then one wave reduction.  Lane 0 of wave 0 gets the
+// results; other threads must not use out.
 template <int K>
-__device__ void xb_part_sum(const XbArgs &a, const int (&f)[K], long long (&out)[K])
+__device__ void xb_part_sum(const XbArgs &a, const int (&f)[K], long long (&out)[K], int ncols)
 {
     if (threadIdx.x >= 64) return;
-    const int ln = xb_lane(), nb = (int)gridDim.x;
+    const int ln = xb_lane();
 #pragma unroll
     for (int k = 0; k < K; k++) {
         long long acc = 0;
-        for (int b = ln; b < nb; b += 64) acc += a.part[(int64_t)f[k] * XB_PMAX + b];
+        for (int b = ln; b < ncols; b += 64) acc += a.part[(int64_t)f[k] * XB_PMAX + b];
         out[k] = (long long)xb_wave_sum_u64((unsigned long long)acc);
     }
 }
 
-// Grid-wide barrier for a cooperative launch (all blocks co-resident).  The
-// wait is bounded: after ~4 s, or when another block reported a timeout, it
-// sets ctl->error and returns false so every block leaves the kernel.
-__device__ bool xb_grid_sync(XbCtl *c)
+// Barrier among the first `members` blocks of a cooperative launch (all
+// blocks co-resident) on the counter / generation pair (cnt, gen).  The wait
+// is bounded: after ~4 s without progress, or when another block reported a
+// timeout, it sets ctl->error and returns false so every block leaves the
+// kernel.  With watch_hb, progress also means a change of ctl->heartbeat
+// (blocks parked during small-frontier levels wait as long as those levels
+// keep coming).
+__device__ bool xb_barrier(XbCtl *c, uint32_t *cnt, uint32_t *genw, uint32_t members, bool watch_hb)
 {
     __shared__ int s_ok;
     __syncthreads();
@@ -105,21 +109,30 @@ __device__ bool xb_grid_sync(XbCtl *c)
         // and one acquire after leaving; the wait itself polls with relaxed
         // loads so it does not invalidate the caches of blocks still working
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const uint32_t nb = gridDim.x;
-        const uint32_t gen = __hip_atomic_load(&c->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t prev =
-            __hip_atomic_fetch_add(&c->bar_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t gen = __hip_atomic_load(genw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t prev = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         int ok = 1;
-        if (prev == nb - 1) {
+        if (prev == members - 1) {
             // the last arrival acquires every other block's release before
             // publishing the new generation
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            __hip_atomic_store(&c->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&c->bar_gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(genw, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            const unsigned long long t0 = wall_clock64();
-            while (__hip_atomic_load(&c->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            unsigned long long t0 = wall_clock64();
+            uint32_t hb = watch_hb ? __hip_atomic_load(&c->heartbeat, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT)
+                                   : 0u;
+            while (__hip_atomic_load(genw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 __builtin_amdgcn_s_sleep(1);
+                if (watch_hb) {
+                    const uint32_t h = __hip_atomic_load(&c->heartbeat, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+                    if (h != hb) {
+                        hb = h;
+                        t0 = wall_clock64();
+                    }
+                }
                 if (__hip_atomic_load(&c->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
                     wall_clock64() - t0 > XB_BAR_TIMEOUT_TICKS) {
                     __hip_atomic_store(&c->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -133,6 +146,27 @@ __device__ bool xb_grid_sync(XbCtl *c)
     }
     __syncthreads();
     return s_ok != 0;
+}
+
+// Grid-wide barrier (every block of the launch).
+__device__ bool xb_grid_sync(XbCtl *c)
+{
+    return xb_barrier(c, &c->bar_count, &c->bar_gen, gridDim.x, false);
+}
+
+// Barrier of the blocks taking part in the current level: the whole grid, or
+// the first nact blocks during small-frontier levels.
+__device__ bool xb_level_sync(XbCtl *c, int nact)
+{
+    if (nact == (int)gridDim.x) return xb_grid_sync(c);
+    return xb_barrier(c, &c->sbar_count, &c->sbar_gen, (uint32_t)nact, false);
+}
+
+// A block sitting out small-frontier levels: arrives at the grid barrier and
+// waits there until the active blocks arrive too.
+__device__ bool xb_park(XbCtl *c)
+{
+    return xb_barrier(c, &c->bar_count, &c->bar_gen, gridDim.x, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -855,11 +889,6 @@ __device__ void xb_mlp_wave_forward(const float *p, const float *x, float *y)
 // The level loop's block-uniform state lives in LDS (thread 0 updates it,
 // everyone reads it after a barrier) so it does not hold scalar registers
 // across the phases, which are register-bound.
-struct XbLoopShared {
-    XbLoop st;
-    long long n_f, m_f, n_u, m_u, p2_next;
-    int32_t open, mode, htag, used;
-};
 
 __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
 {
@@ -883,13 +912,17 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
     }
     __syncthreads();
     XbCtl *c = a.ctl;
+    // blocks taking part in the current level: the grid, or the first
+    // XB_SMALL_BLOCKS during small-frontier levels (the rest park)
+    int nact = nb;
+    const bool small_ok = nb > XB_SMALL_BLOCKS;
     for (;;) {
         const int32_t L = ls.st.L;
         const unsigned long long t0 = wall_clock64();
         const bool need_sweep = ls.st.need_sweep != 0;
         if (need_sweep) {
-            ph_sweep(a, L, bid, nb, true, true, lds);
-            if (!xb_grid_sync(c)) return;
+            ph_sweep(a, L, bid, nact, true, true, lds);
+            if (!xb_level_sync(c, nact)) return;
         }
         // Level statistics and the decision, by wave 0 alone; one block
         // barrier publishes them.
@@ -898,7 +931,7 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
             const int ff[5] = {need_sweep ? xb_f_sweep(L, 0) : xb_f_ring(L, 0),
                                need_sweep ? xb_f_sweep(L, 1) : xb_f_ring(L, 1),
                                xb_f_ring(L + 1, 0), xb_f_ring(L + 1, 1), xb_f_ring(L + 1, 2)};
-            xb_part_sum<5>(a, ff, sv);
+            xb_part_sum<5>(a, ff, sv, nact);
             if (threadIdx.x == 0) {
                 long long n_u = 0, m_u = 0;
                 const bool open = xb_level_open(ls.st, a, sv[0], sv[1], sv[2], sv[3], &n_u, &m_u);
@@ -938,39 +971,68 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
                                                ls.m_f, a.m, a.alpha, &htag, &used);
                     ls.htag = htag;
                     ls.used = used;
+                    ls.small = small_ok && ls.mode == XBFS_MODE_TD_SF && ls.st.q_valid &&
+                               !need_sweep && ls.p2_next == 0 && ls.n_f <= XB_SMALL_NF &&
+                               ls.m_f <= XB_SMALL_MF;
                 }
             }
         }
         __syncthreads();
-        if (!ls.open) return;
-        const int32_t mode = ls.mode;
-        ph_level_prep(a, L, bid, nb);
-        if (mode != XBFS_MODE_BU && !ls.st.q_valid && !need_sweep) {
-            ph_sweep(a, L, bid, nb, false, false, lds);
-            if (!xb_grid_sync(c)) return;
+        // entering / leaving small-frontier levels (every active block takes
+        // the same branch: the inputs are identical)
+        if (!ls.open || (nact < nb && !ls.small)) {
+            if (nact < nb) {
+                // the parked blocks rejoin: they get the loop state, decision
+                // included, from device memory after the grid barrier
+                if (bid == 0 && threadIdx.x == 0) *a.lsg = ls;
+                if (!xb_grid_sync(c)) return;
+                nact = nb;
+            }
+            if (!ls.open) return;
+        } else if (nact == nb && ls.small) {
+            if (bid >= XB_SMALL_BLOCKS) {
+                if (!xb_park(c)) return;
+                if (threadIdx.x == 0) ls = *a.lsg;
+                __syncthreads();
+                if (!ls.open) return;
+            } else {
+                nact = XB_SMALL_BLOCKS;
+            }
         }
-        if (mode == XBFS_MODE_TD_SF) ph_td<true>(a, L, bid, nb, ls.p2_next == 0, sm);
-        else if (mode == XBFS_MODE_TD_SS) ph_td<false>(a, L, bid, nb, false, sm);
-        else ph_bu(a, L, bid, nb, lds);
-        if (!xb_grid_sync(c)) return;
+        // the level itself, on the active blocks; a parked block that just
+        // rejoined takes the level from the copied state
+        const int32_t Lc = ls.st.L;
+        const int32_t mode = ls.mode;
+        ph_level_prep(a, Lc, bid, nact);
+        if (mode != XBFS_MODE_BU && !ls.st.q_valid && !ls.st.need_sweep) {
+            ph_sweep(a, Lc, bid, nact, false, false, lds);
+            if (!xb_level_sync(c, nact)) return;
+        }
+        if (mode == XBFS_MODE_TD_SF) ph_td<true>(a, Lc, bid, nact, ls.p2_next == 0, sm);
+        else if (mode == XBFS_MODE_TD_SS) ph_td<false>(a, Lc, bid, nact, false, sm);
+        else ph_bu(a, Lc, bid, nact, lds);
+        if (!xb_level_sync(c, nact)) return;
         if (threadIdx.x == 0) {
-            if (bid == 0 && L < a.rec_cap) {
-                XbDevRec &r = a.recs[L];
-                r.level = L;
-                r.mode = mode;
-                r.tag = xbfs_mode_to_tag(mode);
-                r.htag = (ls.htag & 0xff) | (ls.used << 8);
-                r.n_f = ls.n_f;
-                r.m_f = ls.m_f;
-                r.n_u = ls.n_u;
-                r.m_u = ls.m_u;
-                r.t0 = (long long)t0;
-                r.t1 = (long long)wall_clock64();
+            if (bid == 0) {
+                if (nact < nb) xb_st(&c->heartbeat, (uint32_t)Lc);
+                if (Lc < a.rec_cap) {
+                    XbDevRec &r = a.recs[Lc];
+                    r.level = Lc;
+                    r.mode = mode;
+                    r.tag = xbfs_mode_to_tag(mode);
+                    r.htag = (ls.htag & 0xff) | (ls.used << 8);
+                    r.n_f = ls.n_f;
+                    r.m_f = ls.m_f;
+                    r.n_u = ls.n_u;
+                    r.m_u = ls.m_u;
+                    r.t0 = (long long)t0;
+                    r.t1 = (long long)wall_clock64();
+                }
             }
             ls.st.q_valid = (mode == XBFS_MODE_TD_SF && ls.p2_next == 0);
             ls.st.need_sweep = (mode == XBFS_MODE_TD_SS);
             ls.st.prev_mode = mode;
-            ls.st.L = L + 1;
+            ls.st.L = Lc + 1;
         }
         __syncthreads();
     }
@@ -1066,6 +1128,9 @@ __global__ void k_ctl_reset(XbCtl *c)
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         xb_st(&c->bar_count, 0u);
         xb_st(&c->bar_gen, 0u);
+        xb_st(&c->sbar_count, 0u);
+        xb_st(&c->sbar_gen, 0u);
+        xb_st(&c->heartbeat, 0u);
         xb_st(&c->error, 0);
     }
 }
