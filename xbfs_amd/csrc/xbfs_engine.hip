@@ -585,15 +585,22 @@ __device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue,
         for (int64_t j = b + threadIdx.x; j < e; j += blockDim.x)
             td_visit<SF>(a, t, u, a.adj[j], true, my_n, my_m);
     }
+    // Wave and thread bins: work is dealt to blocks round robin (wave-bin
+    // vertex i -> block i mod nb; thread-bin entries in runs of 64, run r ->
+    // block r mod nb, so a wave still takes 64 consecutive entries), so a
+    // frontier of a few thousand vertices spreads over every CU instead of
+    // filling the first few.
     // wave bin: one wave per vertex
-    for (int64_t i = gtid >> 6; i < t.nc[1]; i += nthr >> 6) {
+    const int64_t gwi = (int64_t)(threadIdx.x >> 6) * nb + bid;
+    for (int64_t i = gwi; i < t.nc[1]; i += nthr >> 6) {
         const int32_t u = t.qc[1][i];
         const int64_t b = a.beg[u], e = a.beg[u + 1];
         for (int64_t j = b + ln; j < e; j += XB_WAVE)
             td_visit<SF>(a, t, u, a.adj[j], true, my_n, my_m);
     }
     // thread bin: one lane per vertex
-    for (int64_t i = gtid; i < t.nc[0]; i += nthr) {
+    const int64_t gti = (((int64_t)(threadIdx.x >> 6) * nb + bid) << 6) + ln;
+    for (int64_t i = gti; i < t.nc[0]; i += nthr) {
         const int32_t u = t.qc[0][i];
         const int64_t b = a.beg[u], e = a.beg[u + 1];
         for (int64_t j = b; j < e; j++) td_visit<SF>(a, t, u, a.adj[j], true, my_n, my_m);
