@@ -373,18 +373,18 @@ __device__ void ph_sweep(const XbArgs &a, int32_t L, int bid, int nb, bool from_
         const int64_t w = w0 + ln;
         uint64_t myword = 0;
         if (from_levels) {
-            for (int k0 = 0; k0 < XB_CHUNK_WORDS; k0 += 8) {
-                int32_t lv[8];
+            // all 16 level words of the chunk in flight at once (each load
+            // instruction reads 64 consecutive levels)
+            int32_t lv[XB_CHUNK_WORDS];
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const int64_t v = (w0 + k0 + k) * 64 + ln;
-                    lv[k] = v < a.n ? a.level[v] : -2;
-                }
+            for (int k = 0; k < XB_CHUNK_WORDS; k++) {
+                const int64_t v = (w0 + k) * 64 + ln;
+                lv[k] = v < a.n ? a.level[v] : -2;
+            }
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const unsigned long long f = __ballot(lv[k] == L);
-                    if (ln == k0 + k) myword = f;
-                }
+            for (int k = 0; k < XB_CHUNK_WORDS; k++) {
+                const unsigned long long f = __ballot(lv[k] == L);
+                if (ln == k) myword = f;
             }
             if (ln < XB_CHUNK_WORDS && w < a.nwords) {
                 F[w] = myword;
