@@ -108,3 +108,33 @@ def test_traversal_run_and_outputs(tmp_path, env):
         name, td, bu, tot = line.split(";")
         assert name == "rmat16" and abs(float(td) + float(bu) - float(tot)) < 1e-6 * max(1, float(tot))
     assert open(tmp_path / "res.txt").read() == "\n\n"
+
+
+@pytest.mark.gpu
+def test_per_level_records_csv(tmp_path):
+    """XBFS_RECORDS_OUT: one CSV row per level with the frontier statistics;
+    n_f and m_f of each level equal the oracle's vertex and degree counts of
+    that level (the reference's count[L], bfs_adaptiveFQ_wb_async.cuh:1042-1046)."""
+    import csv
+    g = Graph.rmat(14, 16, seed=1014)
+    beg, csr, w = graph_files(tmp_path, g)
+    src = int(np.flatnonzero(g.degrees() > 0)[3])
+    out = tmp_path / "records.csv"
+    for pers in ("1", "0"):
+        p = run([beg, csr, w, 256, 256, 0, "res.txt", 0.1, src, "rmat14", 1], tmp_path,
+                {"XBFS_RECORDS_OUT": str(out), "XBFS_PERSISTENT": pers})
+        assert p.returncode == 0, p.stdout + p.stderr
+    rows = list(csv.DictReader(open(out)))
+    assert list(rows[0].keys()) == ["dataset", "src", "level", "mode", "tag", "n_f", "m_f", "n_u",
+                                    "m_u", "ratio", "t_s"]
+    want, _, _ = O.bfs_queue(g.beg_pos, g.adj, src)
+    cnt, nv, _ = O.level_counts(g.beg_pos, want)
+    runs = [r for r in rows if r["level"] == "0"]
+    assert len(runs) == 2
+    for r in rows:
+        L = int(r["level"])
+        assert r["dataset"] == "rmat14" and int(r["src"]) == src
+        assert int(r["n_f"]) == nv[L] and int(r["m_f"]) == cnt[L]
+        assert int(r["tag"]) == {0: 0, 1: 1, 2: -1}[int(r["mode"])]
+        assert abs(float(r["ratio"]) - cnt[L] / g.m) < 1e-6
+        assert float(r["t_s"]) > 0

@@ -208,7 +208,7 @@ def test_repeat_is_deterministic(rmat10):
 def test_device_scan_matches_exclusive_scan(rmat10):
     g, ctx, d = rmat10
     rng = np.random.default_rng(11)
-    for n in (1, 2, 63, 4095, 4096, 4097, 100_003, 3_000_000):
+    for n in (1, 2, 63, 4095, 4096, 4097, 100_003, 3_000_000, (1 << 24) - 3, 1 << 24):
         say("scan", n)
         x = rng.integers(0, 1 << 20, n, dtype=np.int32)
         assert (ctx.exclusive_scan(x) == O.exclusive_scan(x)).all()

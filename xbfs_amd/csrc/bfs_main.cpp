@@ -20,6 +20,9 @@
 //   XBFS_POLICY      heuristic|learned|td_sf|td_ss|bu       (default heuristic)
 //   XBFS_PERSISTENT  1 = device-side level loop (default), 0 = host-driven
 //   XBFS_LEVELS_OUT  path: write the int32 level array (raw, n entries)
+//   XBFS_RECORDS_OUT path: append the per-level records as CSV, one row per
+//                    level after a header line when the file is new:
+//                    dataset,src,level,mode,tag,n_f,m_f,n_u,m_u,ratio,t_s
 //   XBFS_POLICY_WEIGHTS path: float32 MLP weights for the learned policy
 //                    (default: data/policy_mlp.f32 beside the executable)
 #include <cstdint>
@@ -200,6 +203,20 @@ int main(int argc, char **argv)
     std::ofstream tf("time_Baseline+adaptive+FQ+wb+async_P6000_Camera_more_synch.csv", std::ios::app);
     std::cout << dataset << ";" << "Total time of traversal (s)=" << st.td_s + st.bu_s << std::endl;
     tf << dataset << ";" << st.td_s << ";" << st.bu_s << ";" << st.td_s + st.bu_s << std::endl;
+    if (const char *p = getenv("XBFS_RECORDS_OUT")) {
+        FILE *f = fopen(p, "a+");
+        if (f) {
+            fseek(f, 0, SEEK_END);
+            if (ftell(f) == 0) fprintf(f, "dataset,src,level,mode,tag,n_f,m_f,n_u,m_u,ratio,t_s\n");
+            for (int L = 0; L < st.n_levels && L < (int)rec.size(); L++) {
+                const xbfs_level_rec &r = rec[(size_t)L];
+                fprintf(f, "%s,%d,%d,%d,%d,%lld,%lld,%lld,%lld,%.9g,%.9g\n", dataset.c_str(), src,
+                        r.level, r.mode, r.tag, (long long)r.n_f, (long long)r.m_f, (long long)r.n_u,
+                        (long long)r.m_u, (double)r.ratio, r.t_s);
+            }
+            fclose(f);
+        }
+    }
     if (const char *p = getenv("XBFS_LEVELS_OUT")) {
         FILE *f = fopen(p, "wb");
         if (f) {
