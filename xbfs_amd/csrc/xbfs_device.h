@@ -70,30 +70,6 @@ __device__ __forceinline__ long long xb_wave_incl_scan_i64(long long x)
     return x;
 }
 
-// Block-wide sum of up to 4 u64 counters; thread 0 adds the totals to dst[k]
-// (skipping zeros).  Every thread of the block must call it.
-template <int K>
-__device__ __forceinline__ void xb_block_add_u64(unsigned long long (&v)[K],
-                                                 unsigned long long *const (&dst)[K])
-{
-    __shared__ unsigned long long sh[K][16];
-    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        unsigned long long s = xb_wave_sum_u64(v[k]);
-        if (xb_lane() == 0) sh[k][w] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            unsigned long long t = 0;
-            for (int i = 0; i < nw; i++) t += sh[k][i];
-            if (t) atomicAdd(dst[k], t);
-        }
-    }
-}
-
 // Block-wide exclusive scan of one int64 per thread; returns the exclusive
 // prefix and writes the block total to *total (all threads).
 __device__ __forceinline__ long long xb_block_excl_scan_i64(long long x, long long *total)

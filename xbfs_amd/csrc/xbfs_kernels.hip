@@ -1,4 +1,4 @@
-// Utility kernels of libxbfs (gfx950, wave64): fills, the degree bitmap,
+// Utility kernels of libxbfs (gfx950, wave64): the degree bitmap,
 // the grid-wide ordered unvisited list and the generic device scan.
 //   k_bu_count / k_scan_tiles / k_bu_scatter
 //                   ascending list of unvisited vertices by 64-bit ballot +
@@ -12,19 +12,6 @@
 #include <stdint.h>
 #include "xbfs_device.h"
 #include "xbfs_kernels.h"
-
-// ---------------------------------------------------------------------------
-// init / seed
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_fill_i32(int32_t *__restrict__ a, int64_t n, int32_t val)
-{
-    const int64_t n4 = n >> 2;
-    int4 v4 = make_int4(val, val, val, val);
-    int4 *a4 = reinterpret_cast<int4 *>(a);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) a4[i] = v4;
-    if (blockIdx.x == 0 && threadIdx.x < (n & 3)) a[n4 * 4 + threadIdx.x] = val;
-}
 
 // nz bitmap: bit v set iff deg(v) > 0.  One wave per 64-vertex word.
 __global__ void __launch_bounds__(256) k_build_nz(const int64_t *__restrict__ beg, int64_t n,
