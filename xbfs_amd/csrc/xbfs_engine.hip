@@ -79,18 +79,32 @@ __device__ void xb_part_add(const XbArgs &a, int bid, unsigned long long (&v)[K]
 // (the blocks that wrote them; the zeroing schedule clears every column, so
 // the others are zero), computed by wave 0 alone: every lane issues all its
 // loads at once, then one wave reduction.  Lane 0 of wave 0 gets the
-// results; other threads must not use out.
+// results; other threads must not use out.  The columns are read 256 at a
+// time (4 per lane and field, unrolled), so up to 4K loads per lane are in
+// flight before the first wait: one memory round trip per 256 blocks instead
+// of one per column.
 template <int K>
 __device__ void xb_part_sum(const XbArgs &a, const int (&f)[K], long long (&out)[K], int ncols)
 {
     if (threadIdx.x >= 64) return;
     const int ln = xb_lane();
+    long long acc[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        long long acc = 0;
-        for (int b = ln; b < ncols; b += 64) acc += a.part[(int64_t)f[k] * XB_PMAX + b];
-        out[k] = (long long)xb_wave_sum_u64((unsigned long long)acc);
+    for (int k = 0; k < K; k++) acc[k] = 0;
+    for (int b0 = 0; b0 < ncols; b0 += 256) {
+        long long v[K][4];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int b = b0 + ln + 64 * j;
+                v[k][j] = b < ncols ? a.part[(int64_t)f[k] * XB_PMAX + b] : 0;
+            }
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] += (v[k][0] + v[k][1]) + (v[k][2] + v[k][3]);
     }
+#pragma unroll
+    for (int k = 0; k < K; k++) out[k] = (long long)xb_wave_sum_u64((unsigned long long)acc[k]);
 }
 
 // Barrier among the first `members` blocks of a cooperative launch (all
