@@ -681,14 +681,17 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
         xb_wave_sync();
         // Hub probe, 4 candidates per lane per round (independent loads): a
         // candidate whose largest-degree neighbour is in F_L gets level L+1
-        // with that neighbour as parent, without reading its adjacency.  The
-        // others are compacted to the front of the list (positions below the
-        // entries read so far) for the scan below.
+        // with that neighbour as parent, without reading its adjacency.  For
+        // a degree-1 candidate the hub is its only neighbour, so the probe
+        // settles it completely: on a miss, the hub's F_{L+1} bit (loaded
+        // with the F_L word) applies the +2 rule, and either way it needs no
+        // scan.  The others are compacted to the front of the list (positions
+        // below the entries read so far) for the scan below.
         int nres = 0;
         for (int r = 0; r < total; r += 256) {
             int32_t vc[4];
             int2 h[4];
-            bool hit[4];
+            bool hit[4], done[4];
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 const int i = r + 64 * t + ln;
@@ -696,27 +699,38 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
             }
 #pragma unroll
             for (int t = 0; t < 4; t++) h[t] = vc[t] >= 0 ? a.hd[vc[t]] : make_int2(-1, 0);
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-                hit[t] = h[t].x >= 0 && ((F[h[t].x >> 6] >> (h[t].x & 63)) & 1ull);
+            uint64_t fw[4], f1w[4];
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                if (hit[t]) {
+                fw[t] = h[t].x >= 0 ? F[h[t].x >> 6] : 0ull;
+                f1w[t] = (h[t].x >= 0 && h[t].y == 1) ? F1[h[t].x >> 6] : 0ull;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                hit[t] = (fw[t] >> (h[t].x & 63)) & 1ull;
+                done[t] = hit[t] || h[t].y == 1;
+                const bool p2 = !hit[t] && h[t].y == 1 && ((f1w[t] >> (h[t].x & 63)) & 1ull);
+                if (hit[t] || p2) {
                     const int32_t v = vc[t];
                     const unsigned long long bit = 1ull << (v & 63);
                     const int li = (int)((v >> 6) - w0);
-                    a.level[v] = L1;
+                    a.level[v] = hit[t] ? L1 : L2;
                     if (a.parent) a.parent[v] = h[t].x;
-                    atomicOr(&wl->f1[li], bit);
+                    atomicOr(hit[t] ? &wl->f1[li] : &wl->f2[li], bit);
                     atomicOr(&wl->va[li], bit);
-                    n1 += 1;
-                    m1 += (unsigned long long)h[t].y;
+                    if (hit[t]) {
+                        n1 += 1;
+                        m1 += (unsigned long long)h[t].y;
+                    } else {
+                        n2 += 1;
+                        m2 += 1;
+                    }
                 }
             }
             xb_wave_sync();
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const bool un = vc[t] >= 0 && !hit[t];
+                const bool un = vc[t] >= 0 && !done[t];
                 const unsigned long long um = __ballot(un);
                 if (un) wl->list[nres + xb_rank(um)] = vc[t];
                 nres += (int)__popcll(um);
