@@ -22,6 +22,11 @@ def test_features_closed_form():
     # log2(1+v) approximation stays within 1e-4 of the true value
     for v in (2, 3, 6, 100, 12345, 10 ** 9, 2 ** 40 + 12345):
         assert abs(C.policy_features(0, v, 0, 0, 0, 1)[0] * 32 - np.log2(1 + v)) < 1e-4
+    # on both sides of every power of two (the exponent comes from the top bit of 1+v)
+    for k in range(1, 45):
+        for v in (2 ** k - 2, 2 ** k - 1, 2 ** k):
+            if v >= 0:
+                assert abs(C.policy_features(0, v, 0, 0, 0, 1)[0] * 32 - np.log2(1 + v)) < 1e-4, v
 
 
 def test_train_specs_are_seeded_and_in_range():
@@ -102,6 +107,37 @@ def test_probe_records_and_learned_path_exact():
             want, _, _ = O.bfs_queue(g.beg_pos, g.adj, int(s))
             assert (lv == want).all()
             assert O.validate(g.beg_pos, g.adj, int(s), lv, par) == 0
+
+
+@pytest.mark.gpu
+def test_persistent_decisions_follow_the_model():
+    """k_persistent evaluates the model with one wave from weights laid out
+    transposed in LDS: at every level its choice is the argmin of the same
+    model evaluated by xbfs_mlp_forward on that level's features (levels
+    whose two best predictions lie within 1e-3 are skipped as ties)."""
+    g = Graph.rmat(18, 16, seed=5)
+    ctx = Context(g)
+    srcs = P.seeded_sources(g.degrees(), 4, seed=3)
+    rng = np.random.default_rng(2)
+    weights = [(rng.standard_normal(C.MLP_NPARAM) * 0.5).astype(np.float32)]
+    if os.path.exists(P.WEIGHTS):
+        weights.append(P.load())
+    checked = 0
+    for w in weights:
+        ctx.load_policy(w)
+        for s in srcs:
+            _, _, recs, _ = ctx.bfs(int(s), want_levels=False, rec_cap=64, policy=POLICY_LEARNED)
+            X = np.stack([C.policy_features(r["level"], r["n_f"], r["m_f"], r["n_u"], r["m_u"],
+                                            recs[i - 1]["mode"] if i else -1)
+                          for i, r in enumerate(recs)])
+            pred = C.mlp_forward(w, X)
+            for r, y in zip(recs, pred):
+                srt = np.sort(y)
+                if r["policy"] != POLICY_LEARNED or srt[1] - srt[0] < 1e-3:
+                    continue
+                assert r["mode"] == int(np.argmin(y)), (s, r, y)
+                checked += 1
+    assert checked >= 8
 
 
 @pytest.mark.gpu

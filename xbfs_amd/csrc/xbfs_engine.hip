@@ -895,29 +895,44 @@ __host__ __device__ inline bool xb_level_open(XbLoop &st, const XbArgs &a, long 
     return true;
 }
 
-// MLP forward by one wave from LDS weights p (packed as in xbfs_policy.h):
+// LDS position of packed MLP parameter k (packed as in xbfs_policy.h): the
+// weight matrices are stored transposed (W1T[i][o], W2T[i][o], W3T[i][o]),
+// so lanes o = 0..31 reading input i touch 32 consecutive words (no bank
+// conflicts); biases keep their places.
+__device__ __forceinline__ int xb_mlp_lds_pos(int k)
+{
+    const int nw1 = XB_MLP_NH * XB_MLP_NI, o2 = nw1 + XB_MLP_NH;
+    const int nw2 = XB_MLP_NH * XB_MLP_NH, o3 = o2 + nw2 + XB_MLP_NH;
+    if (k < nw1) return (k % XB_MLP_NI) * XB_MLP_NH + k / XB_MLP_NI;
+    if (k >= o2 && k < o2 + nw2) return o2 + ((k - o2) % XB_MLP_NH) * XB_MLP_NH + (k - o2) / XB_MLP_NH;
+    if (k >= o3 && k < o3 + XB_MLP_NO * XB_MLP_NH)
+        return o3 + ((k - o3) % XB_MLP_NH) * XB_MLP_NO + (k - o3) / XB_MLP_NH;
+    return k;
+}
+
+// MLP forward by one wave from LDS weights p (laid out by xb_mlp_lds_pos):
 // lane o < 32 computes hidden unit o, the hidden vectors move between lanes
 // by shuffles, lanes 0..2 write the outputs to y.  Sums run in input order,
 // as in xbfs_mlp_forward1.  Called by every lane of the wave.
 __device__ void xb_mlp_wave_forward(const float *p, const float *x, float *y)
 {
-    const float *W1 = p, *b1 = W1 + XB_MLP_NH * XB_MLP_NI;
-    const float *W2 = b1 + XB_MLP_NH, *b2 = W2 + XB_MLP_NH * XB_MLP_NH;
-    const float *W3 = b2 + XB_MLP_NH, *b3 = W3 + XB_MLP_NO * XB_MLP_NH;
+    const float *W1T = p, *b1 = W1T + XB_MLP_NH * XB_MLP_NI;
+    const float *W2T = b1 + XB_MLP_NH, *b2 = W2T + XB_MLP_NH * XB_MLP_NH;
+    const float *W3T = b2 + XB_MLP_NH, *b3 = W3T + XB_MLP_NO * XB_MLP_NH;
     const int o = xb_lane();
     const int oh = o < XB_MLP_NH ? o : 0, oo = o < XB_MLP_NO ? o : 0;
     // fully unrolled: every weight load and every lane read is issued up front
     float t1 = b1[oh];
 #pragma unroll
-    for (int i = 0; i < XB_MLP_NI; i++) t1 += W1[oh * XB_MLP_NI + i] * x[i];
+    for (int i = 0; i < XB_MLP_NI; i++) t1 += W1T[i * XB_MLP_NH + oh] * x[i];
     const float h1 = (o < XB_MLP_NH && t1 > 0.0f) ? t1 : 0.0f;
     float t2 = b2[oh];
 #pragma unroll
-    for (int i = 0; i < XB_MLP_NH; i++) t2 += W2[oh * XB_MLP_NH + i] * __shfl(h1, i);
+    for (int i = 0; i < XB_MLP_NH; i++) t2 += W2T[i * XB_MLP_NH + oh] * __shfl(h1, i);
     const float h2 = (o < XB_MLP_NH && t2 > 0.0f) ? t2 : 0.0f;
     float t3 = b3[oo];
 #pragma unroll
-    for (int i = 0; i < XB_MLP_NH; i++) t3 += W3[oo * XB_MLP_NH + i] * __shfl(h2, i);
+    for (int i = 0; i < XB_MLP_NH; i++) t3 += W3T[i * XB_MLP_NO + oo] * __shfl(h2, i);
     if (o < XB_MLP_NO) y[o] = t3;
 }
 
@@ -930,11 +945,11 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
     const int bid = blockIdx.x, nb = gridDim.x;
     __shared__ XbSmem sm;
     __shared__ XbLoopShared ls;
-    __shared__ float s_mlp[XB_MLP_NPARAM]; // learned policy weights, loaded once
+    __shared__ float s_mlp[XB_MLP_NPARAM]; // learned policy weights, loaded once (transposed)
     extern __shared__ __align__(16) unsigned char xb_dyn_lds[];
     XbWaveLds *lds = reinterpret_cast<XbWaveLds *>(xb_dyn_lds);
     if (a.mlp)
-        for (int i = threadIdx.x; i < XB_MLP_NPARAM; i += blockDim.x) s_mlp[i] = a.mlp[i];
+        for (int i = threadIdx.x; i < XB_MLP_NPARAM; i += blockDim.x) s_mlp[xb_mlp_lds_pos(i)] = a.mlp[i];
     ph_init(a, bid, nb);
     if (!xb_grid_sync(a.ctl)) return;
     if (a.nsrc > 0) {

@@ -122,12 +122,16 @@ XBFS_HD inline int32_t xbfs_mode_to_tag(int32_t mode)
 
 XBFS_HD inline float xb_log2p1(long long v)
 {
-    // log2(1+v) without libm: the exponent from repeated halving, the
-    // mantissa part by a degree-5 least-squares fit of log2(1+f) on [0,1]
-    // that is exact at f = 0 and f = 1 (max error 2.2e-5; features only)
+    // log2(1+v) without libm: the exponent from the top set bit of 1+v (the
+    // power-of-two scaling to [1,2) is exact), the mantissa part by a
+    // degree-5 least-squares fit of log2(1+f) on [0,1] that is exact at
+    // f = 0 and f = 1 (max error 2.2e-5; features only)
     double x = (double)v + 1.0;
     int e = 0;
-    while (x >= 2.0) { x *= 0.5; e++; }
+    if (v >= 1) {
+        e = 63 - __builtin_clzll((unsigned long long)v + 1ull);
+        x *= 1.0 / (double)(1ull << e);
+    }
     const double f = x - 1.0;
     const double l =
         f * (1.4417403 + f * (-0.70777018 + f * (0.41234422 + f * (-0.19031903 + f * 0.04400469))));
