@@ -20,6 +20,35 @@ def csr_from_edges(n, edges, symmetric=True):
     return beg, v.astype(np.int32)
 
 
+def raw_csr(n, pairs):
+    """CSR straight from directed pairs, keeping duplicates and self-loops
+    (rows sorted): the loader accepts such files as they are."""
+    e = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    e = e[np.lexsort((e[:, 1], e[:, 0]))]
+    beg = np.zeros(n + 1, np.int64)
+    np.add.at(beg, e[:, 0] + 1, 1)
+    return np.cumsum(beg), e[:, 1].astype(np.int32)
+
+
+def loops_and_duplicates():
+    """Self-loops, duplicate entries and degree-1 chains around a small core:
+    every edge of the path 0..19 twice; 20, 21 with only a self-loop; 22 with
+    a self-loop and an edge to 0; the pair 23-24; a star on 25 (leaves 26..60,
+    even leaves joined twice); 61-62-63 hanging off leaf 26 (61 and 63 have
+    degree 1 after 62 gets a self-loop too); 64 joined to 19 twice and to
+    itself."""
+    und = []
+    for i in range(19):
+        und += [(i, i + 1), (i, i + 1)]
+    und += [(22, 0), (23, 24)]
+    for leaf in range(26, 61):
+        und += [(25, leaf)] * (2 if leaf % 2 == 0 else 1)
+    und += [(26, 61), (61, 62), (62, 63), (64, 19), (64, 19)]
+    pairs = [(u, v) for u, v in und] + [(v, u) for u, v in und]
+    pairs += [(20, 20), (21, 21), (22, 22), (62, 62), (64, 64)]
+    return raw_csr(65, pairs)
+
+
 def path(n):
     return csr_from_edges(n, [(i, i + 1) for i in range(n - 1)])
 

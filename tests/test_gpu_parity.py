@@ -73,6 +73,7 @@ CLOSED = {
     "complete": lambda: G.complete(70),
     "disjoint": lambda: G.disjoint(G.path(40), G.star(100), G.csr_from_edges(7, []),
                                    G.grid(9, 9)),
+    "loops_dups": G.loops_and_duplicates,
 }
 POLICIES = [dict(policy=POLICY_HEURISTIC), dict(policy=POLICY_FORCE_TD_SF),
             dict(policy=POLICY_FORCE_TD_SS), dict(policy=POLICY_FORCE_BU),

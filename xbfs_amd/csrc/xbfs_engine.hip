@@ -284,16 +284,20 @@ __device__ void ph_seed(const XbArgs &a, int bid, int nb)
 // Zeroing schedule at the start of level L (everything zeroed here is
 // first written at level L+1): queue buffer (L+2)%3 (block 0), the partial
 // statistics of ring slot (L+3)&3 and sweep slot (L+1)&1, and bitmap ring
-// (L+3)&3 = (L-1)&3, which no phase of level L reads or writes.
-__device__ void ph_level_prep(const XbArgs &a, int32_t L, int bid, int nb)
+// (L+3)&3 = (L-1)&3, which no phase of level L reads or writes (nor the
+// level's statistics and decision, so the two may run side by side).
+// Threads below t_lo of each block take no part.
+__device__ void ph_level_prep(const XbArgs &a, int32_t L, int bid, int nb, int t_lo = 0)
 {
-    if (bid == 0 && threadIdx.x == 0) {
+    if ((int)threadIdx.x < t_lo) return;
+    const int tid = (int)threadIdx.x - t_lo, bdim = (int)blockDim.x - t_lo;
+    if (bid == 0 && tid == 0) {
         XbCtl *c = a.ctl;
         const int qb = (L + 2) % XB_QBUFS;
         for (int k = 0; k < XB_BINS; k++) xb_st(&c->qcnt[qb][k].v, 0);
     }
-    const int64_t gtid = (int64_t)bid * blockDim.x + threadIdx.x;
-    const int64_t nthr = (int64_t)nb * blockDim.x;
+    const int64_t gtid = (int64_t)bid * bdim + tid;
+    const int64_t nthr = (int64_t)nb * bdim;
     const int zf[5] = {xb_f_ring(L + 3, 0), xb_f_ring(L + 3, 1), xb_f_ring(L + 3, 2),
                        xb_f_sweep(L + 1, 0), xb_f_sweep(L + 1, 1)};
     for (int64_t i = gtid; i < 5 * XB_PMAX; i += nthr)
@@ -974,8 +978,11 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
             ph_sweep(a, L, bid, nact, true, true, lds);
             if (!xb_level_sync(c, nact)) return;
         }
-        // Level statistics and the decision, by wave 0 alone; one block
-        // barrier publishes them.
+        // Level statistics and the decision, by wave 0 alone, while the
+        // other waves zero what level L+1 will write; one block barrier
+        // publishes the decision.  Blocks that sit out small-frontier levels
+        // and rejoin later skip this level's zeroing: the active blocks did it.
+        if (threadIdx.x >= 64) ph_level_prep(a, L, bid, nact, 64);
         if (threadIdx.x < 64) {
             long long sv[5];
             const int ff[5] = {need_sweep ? xb_f_sweep(L, 0) : xb_f_ring(L, 0),
@@ -1053,7 +1060,6 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
         // rejoined takes the level from the copied state
         const int32_t Lc = ls.st.L;
         const int32_t mode = ls.mode;
-        ph_level_prep(a, Lc, bid, nact);
         if (mode != XBFS_MODE_BU && !ls.st.q_valid && !ls.st.need_sweep) {
             ph_sweep(a, Lc, bid, nact, false, false, lds);
             if (!xb_level_sync(c, nact)) return;
