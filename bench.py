@@ -22,7 +22,9 @@ by the device time of the launch that runs the traversal, against the
 8 TB/s HBM peak.
 
 cpu_baseline: OpenMP top-down BFS (the oracle's CPU baseline) on a bounded
-sample of the same sources, on this host's cores.
+sample of the same sources, on this host's cores.  cpu_serial: the
+reference's own CPU path (the level-synchronous scan BFS of
+bfs_gpu_update.cu:104-127, restated in the oracle) on one core, first source.
 
 batch64 (reported beside the headline, --batch 1): RMAT scale 22 ef16 (seed
 1022), 64 seeded sources; device time of one 64-source bit-parallel batch
@@ -137,6 +139,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4,
                     help="sources timed for the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-serial", type=int, default=1,
+                    help="sources timed for the single-core serial reference path (0 = skip)")
     ap.add_argument("--oracle-best", type=int, default=16,
                     help="sources on which the per-level oracle-best schedule is measured")
     ap.add_argument("--compare", type=int, default=1,
@@ -253,6 +257,17 @@ def main():
             cg.append(float(deg[lvc >= 0].sum()) / tc / 1e9)
         cpu = {"value": hmean(cg), "unit": "GTEPS", "cores": nthr, "kind": "port",
                "sample": f"OpenMP top-down BFS, first {len(cg)} of the 64 sources, same graph"}
+    cpu_serial = None
+    if args.cpu_serial > 0:
+        cs = []
+        for s in srcs[: args.cpu_serial]:
+            t0 = time.perf_counter()
+            lvs, _ = O.bfs_scan(g.beg_pos, g.adj, int(s))
+            ts = time.perf_counter() - t0
+            cs.append(float(deg[lvs >= 0].sum()) / ts / 1e9)
+        cpu_serial = {"value": hmean(cs), "unit": "GTEPS", "cores": 1, "kind": "port",
+                      "sample": f"reference serial scan BFS (bfs_gpu_update.cu:104-127), "
+                                f"first {len(cs)} of the 64 sources, same graph"}
 
     b64 = mg = None
     if args.batch:
@@ -290,6 +305,7 @@ def main():
                      "launches_per_bfs": 1 if opts.persistent else None,
                      "avg_launch_us": round(float(t_dev[main_pol].mean()) * 1e6, 2)},
         "cpu_baseline": cpu,
+        "cpu_serial": cpu_serial,
         "batch64": b64,
         "multigraph": mg,
         "gteps_min": round(float(gteps.min()), 3),
