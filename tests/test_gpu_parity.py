@@ -93,6 +93,27 @@ def test_closed_forms_all_policies(name):
     ctx.close()
 
 
+def test_csr_row_order(monkeypatch):
+    """XBFS_ROW_ORDER=csr keeps the uploaded adjacency order (no degree sort,
+    bottom-up scans from each row's first entry): same levels, valid parents."""
+    monkeypatch.setenv("XBFS_ROW_ORDER", "csr")
+    for name in ("loops_dups", "star", "disjoint"):
+        g = Graph(*CLOSED[name]())
+        ctx = Context(g)
+        for src in (0, g.n // 2, g.n - 1):
+            for pol in POLICIES[:4]:
+                say("csr order", name, src, pol)
+                run_check(ctx, g, src, **pol)
+        ctx.close()
+    g = Graph.rmat(14, 16, seed=1414)
+    ctx = Context(g)
+    for s in seeded_sources(g, 4):
+        for pol in POLICIES[:4]:
+            say("csr order rmat14", s, pol)
+            run_check(ctx, g, s, **pol)
+    ctx.close()
+
+
 def test_degree_zero_source():
     beg, adj = G.disjoint(G.path(10), G.csr_from_edges(3, []))
     g = Graph(beg, adj)

@@ -5,9 +5,12 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
+
+#include <rocprim/device/device_segmented_radix_sort.hpp>
 
 #include "xbfs.h"
 #include "xbfs_policy.h"
@@ -48,8 +51,9 @@ bool pow2_block(int b) { return b >= 64 && b <= 1024 && (b & (b - 1)) == 0; }
 int xb_ctx_free_device(xbfs_ctx *c)
 {
     (void)hipSetDevice(c->device);
-    void *bufs[] = {c->d_beg,   c->d_adj,  c->d_nz,    c->d_hd,   c->d_vis,  c->d_fbm,   c->d_level,
-                    c->d_parent, c->d_qbuf, c->d_ctl,  c->d_lsg, c->d_recs, c->d_sched, c->d_mlp,
+    void *bufs[] = {c->d_beg,   c->d_adj,  c->d_adj_bu, c->d_nz,  c->d_hd,   c->d_vis,
+                    c->d_fbm,   c->d_level, c->d_parent, c->d_qbuf, c->d_ctl,  c->d_lsg,
+                    c->d_recs,  c->d_sched, c->d_mlp,
                     c->d_list,  c->d_ubits, c->d_tile, c->d_list_cnt, c->s_level,
                     c->s_vis,   c->s_fbm,   c->s_qbuf, c->s_ctl, c->d_part, c->s_part,
                     c->d_srcs,  c->d_b64_deg, c->d_b64_segu, c->d_b64_sege, c->d_b64_F,
@@ -91,6 +95,55 @@ void xbfs_opts_default(xbfs_opts *o)
 int64_t xbfs_num_vertices(const xbfs_ctx *c) { return c ? c->n : -1; }
 int64_t xbfs_num_edges(const xbfs_ctx *c) { return c ? c->m : -1; }
 
+// Builds the bottom-up copy of the adjacency, c->d_adj_bu: every row ordered
+// by neighbour degree, descending, keeping the CSR order among equal degrees
+// (stable radix sort on the device).  A row then starts with its hub, so a
+// bottom-up scan that probed the hub first skips it, and the other
+// high-degree neighbours (the likeliest frontier members) come early.  The
+// top-down phases keep the uploaded order: there, a degree-first order sends
+// many lanes to the same few hubs at once.  Levels do not depend on the
+// order; parents stay tree edges.  Returns false (no copy) when the sizes
+// exceed the sort's 32-bit counts or the memory is not available.
+static bool sort_rows_by_degree(xbfs_ctx *c, int64_t max_deg)
+{
+    if (c->m < 2 || c->m > INT32_MAX || c->n > INT32_MAX) return false;
+    int end_bit = 1;
+    while (end_bit < 32 && (max_deg >> end_bit) != 0) end_bit++;
+    uint32_t *kin = nullptr, *kout = nullptr;
+    int32_t *adj2 = nullptr;
+    void *tmp = nullptr;
+    size_t tb = 0;
+    const size_t m = (size_t)c->m;
+    bool ok = rocprim::segmented_radix_sort_pairs_desc(nullptr, tb, kin, kout, c->d_adj, adj2,
+                                                       (unsigned int)m, (unsigned int)c->n,
+                                                       c->d_beg, c->d_beg + 1, 0, end_bit,
+                                                       c->stream) == hipSuccess &&
+              hipMalloc(&kin, sizeof(uint32_t) * m) == hipSuccess &&
+              hipMalloc(&kout, sizeof(uint32_t) * m) == hipSuccess &&
+              hipMalloc(&adj2, sizeof(int32_t) * m) == hipSuccess &&
+              hipMalloc(&tmp, tb > 0 ? tb : 1) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(k_nbr_deg, dim3(grid_for(c->m, 256, 65536)), dim3(256), 0, c->stream,
+                           c->d_beg, c->d_adj, c->m, kin);
+        ok = hipGetLastError() == hipSuccess &&
+             rocprim::segmented_radix_sort_pairs_desc(tmp, tb, kin, kout, c->d_adj, adj2,
+                                                      (unsigned int)m, (unsigned int)c->n,
+                                                      c->d_beg, c->d_beg + 1, 0, end_bit,
+                                                      c->stream) == hipSuccess &&
+             hipStreamSynchronize(c->stream) == hipSuccess;
+    }
+    if (ok) {
+        c->d_adj_bu = adj2;
+        adj2 = nullptr;
+    }
+    if (kin) (void)hipFree(kin);
+    if (kout) (void)hipFree(kout);
+    if (adj2) (void)hipFree(adj2);
+    if (tmp) (void)hipFree(tmp);
+    (void)hipGetLastError(); // a failed allocation must not linger as the next call's error
+    return ok;
+}
+
 int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, int64_t m,
                 xbfs_ctx **out)
 {
@@ -111,8 +164,11 @@ int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, i
     c->ntiles_bu = ceil_div(c->nwords, XB_BU_TILE_WORDS);
     c->fstride = c->ntiles_bu * XB_BU_TILE_WORDS;
     c->ntiles_list = ceil_div(c->nwords, XB_TILE_WORDS);
-    int64_t nz = 0;
-    for (int64_t i = 0; i < n; i++) nz += beg[i + 1] > beg[i];
+    int64_t nz = 0, max_deg = 0;
+    for (int64_t i = 0; i < n; i++) {
+        nz += beg[i + 1] > beg[i];
+        max_deg = std::max<int64_t>(max_deg, beg[i + 1] - beg[i]);
+    }
     c->n_nz = nz;
     auto fail = [&](int code) {
         xb_ctx_free_device(c);
@@ -167,7 +223,15 @@ int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, i
         return fail(XBFS_ERR_HIP);
     hipLaunchKernelGGL(k_build_nz, dim3(grid_for(c->nwords * 64, 256, 65536)), dim3(256), 0,
                        c->stream, c->d_beg, n, c->d_nz, c->nwords);
-    if (n > 0) {
+    // XBFS_ROW_ORDER=csr keeps the uploaded row order (bottom-up scans then
+    // start at the row's first entry)
+    const char *row_order = getenv("XBFS_ROW_ORDER");
+    const bool keep_csr = row_order && strcmp(row_order, "csr") == 0;
+    c->hub_first = (!keep_csr && sort_rows_by_degree(c, max_deg)) ? 1 : 0;
+    if (n > 0 && c->hub_first) {
+        hipLaunchKernelGGL(k_build_hd_sorted, dim3(grid_for(n, 256, 65536)), dim3(256), 0,
+                           c->stream, c->d_beg, c->d_adj_bu, n, c->d_hd);
+    } else if (n > 0) {
         hipLaunchKernelGGL(k_build_hd, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream,
                            c->d_beg, c->d_adj, n, c->d_hd);
         hipLaunchKernelGGL(k_build_hd_big, dim3(c->num_cu * 8), dim3(256), 0, c->stream, c->d_beg,
@@ -229,6 +293,7 @@ static XbArgs make_args(xbfs_ctx *c, int32_t src, const xbfs_opts &o)
     a.parent = o.want_parent ? c->d_parent : nullptr;
     a.beg = c->d_beg;
     a.adj = c->d_adj;
+    a.adj_bu = c->hub_first ? c->d_adj_bu : c->d_adj;
     a.nz = c->d_nz;
     a.hd = c->d_hd;
     a.vis = c->d_vis;
@@ -250,6 +315,7 @@ static XbArgs make_args(xbfs_ctx *c, int32_t src, const xbfs_opts &o)
     a.n_nz = c->n_nz;
     a.srcs = nullptr;
     a.nsrc = 0;
+    a.hub_first = c->hub_first;
     a.src = src;
     a.t1 = o.bin_thread_max;
     a.t2 = o.bin_wave_max;

@@ -26,6 +26,8 @@ struct xbfs_ctx {
     int32_t *d_adj = nullptr;   // int32[m]
     uint64_t *d_nz = nullptr;   // bit v: deg(v) > 0
     int2 *d_hd = nullptr;       // {largest-degree neighbour, degree} per vertex
+    int32_t *d_adj_bu = nullptr; // bottom-up copy of d_adj, rows by neighbour degree (or NULL)
+    int32_t hub_first = 0;      // d_adj_bu is built (each row starts with its hub)
     uint64_t *d_vis = nullptr;  // visited bitmap
     uint64_t *d_fbm = nullptr;  // 4 frontier bitmaps (ring by level & 3)
     int32_t *d_level = nullptr; // int32[n]

@@ -70,6 +70,7 @@ struct XbArgs {
     int32_t *parent;  // NULL when parents are not wanted
     const int64_t *beg;
     const int32_t *adj;
+    const int32_t *adj_bu; // rows by neighbour degree, hub first (hub_first), else adj
     const uint64_t *nz; // bit v: deg(v) > 0
     const int2 *hd;     // {largest-degree neighbour, degree} per vertex
     uint64_t *vis;      // bit v: level[v] != -1
@@ -87,6 +88,7 @@ struct XbArgs {
     int64_t fstride; // words between frontier rings: nwords rounded up to a tile (256 B)
     const int32_t *srcs; // multi-source seeds (device), used when nsrc > 0
     int32_t nsrc;
+    int32_t hub_first; // 1: every row of adj_bu starts with its hub (hd.x)
     int32_t src, t1, t2, t3;
     int32_t policy;
     float alpha;

@@ -747,7 +747,11 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
         // XB_BU_LANE_MAX edges per lane: a candidate still unresolved with
         // edges left is deferred to the front of the list (positions < the
         // consumed count, so no unread entry is overwritten) and finished by
-        // the whole wave below.
+        // the whole wave below.  When every row starts with its hub (rows
+        // sorted by neighbour degree), the scan starts after it: the hub pass
+        // probed it.  Its level-(L+1) bit then goes unchecked, which only
+        // leaves a +2 discovery to the next level.
+        const int64_t skip = a.hub_first ? 1 : 0;
         int ndef = 0, next = 0;
         int32_t v = -1, seen = -1;
         int64_t j = 0, ce = 0, e = 0, b = 0;
@@ -760,8 +764,8 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                     v = wl->list[k];
                     b = a.beg[v];
                     e = a.beg[v + 1];
-                    j = b;
-                    ce = (e - b > XB_BU_LANE_MAX) ? b + XB_BU_LANE_MAX : e;
+                    j = b + skip;
+                    ce = (e - j > XB_BU_LANE_MAX) ? j + XB_BU_LANE_MAX : e;
                     seen = -1;
                 }
             }
@@ -770,10 +774,10 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
             bool defer = false;
             const int32_t vv = v;
             if (v >= 0) {
-                const int32_t u0 = a.adj[j];
-                const int32_t u1 = j + 1 < ce ? a.adj[j + 1] : -1;
-                const int32_t u2 = j + 2 < ce ? a.adj[j + 2] : -1;
-                const int32_t u3 = j + 3 < ce ? a.adj[j + 3] : -1;
+                const int32_t u0 = a.adj_bu[j];
+                const int32_t u1 = j + 1 < ce ? a.adj_bu[j + 1] : -1;
+                const int32_t u2 = j + 2 < ce ? a.adj_bu[j + 2] : -1;
+                const int32_t u3 = j + 3 < ce ? a.adj_bu[j + 3] : -1;
                 const bool h0 = (F[u0 >> 6] >> (u0 & 63)) & 1ull;
                 const bool h1 = u1 >= 0 && ((F[u1 >> 6] >> (u1 & 63)) & 1ull);
                 const bool h2 = u2 >= 0 && ((F[u2 >> 6] >> (u2 & 63)) & 1ull);
@@ -827,12 +831,12 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
             const int32_t v = wl->list[d];
             const int64_t b = a.beg[v], e = a.beg[v + 1];
             int32_t found = -1;
-            for (int64_t base = b + XB_BU_LANE_MAX; base < e; base += 256) {
+            for (int64_t base = b + skip + XB_BU_LANE_MAX; base < e; base += 256) {
                 int32_t u[4];
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
                     const int64_t jj = base + ln + 64 * t;
-                    u[t] = jj < e ? a.adj[jj] : -1;
+                    u[t] = jj < e ? a.adj_bu[jj] : -1;
                 }
                 bool h[4];
 #pragma unroll

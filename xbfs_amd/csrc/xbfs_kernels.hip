@@ -58,6 +58,31 @@ __global__ void __launch_bounds__(256) k_build_hd(const int64_t *__restrict__ be
     }
 }
 
+// Sort keys of the degree-ordered adjacency: key[j] = deg(adj[j]).
+__global__ void __launch_bounds__(256) k_nbr_deg(const int64_t *__restrict__ beg,
+                                                 const int32_t *__restrict__ adj, int64_t m,
+                                                 uint32_t *__restrict__ key)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride) {
+        const int32_t u = adj[j];
+        key[j] = (uint32_t)(beg[u + 1] - beg[u]);
+    }
+}
+
+// Hub table of a degree-ordered adjacency (stable sort, descending): the
+// first entry of a row is its first neighbour of largest degree.
+__global__ void __launch_bounds__(256) k_build_hd_sorted(const int64_t *__restrict__ beg,
+                                                         const int32_t *__restrict__ adj, int64_t n,
+                                                         int2 *__restrict__ hd)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const int64_t b = beg[v], e = beg[v + 1];
+        hd[v] = make_int2(e > b ? adj[b] : -1, (int32_t)(e - b));
+    }
+}
+
 // Marked vertices, one wave each: a wave checks 64 vertices per load, then
 // for each marked one its lanes stride the adjacency and a wave reduction on
 // (degree, -position) keeps the first neighbour of largest degree.
