@@ -53,8 +53,8 @@ HBM_PEAK = 8.0e12
 # WRITE_SIZE in KiB per dispatch, each scaled by what the same counter reads
 # for a plain 1 GiB device copy (FETCH_SIZE reads half of the bytes a copy
 # moves: factor 2.0; WRITE_SIZE reads all of them: factor 1.0).
-PMC_TRAFFIC = {"workload": (24, 16, 64, "learned"), "fetch_kib": 685155.0, "fetch_factor": 1.99995,
-               "write_kib": 266808.0, "write_factor": 1.0}
+PMC_TRAFFIC = {"workload": (24, 16, 64, "learned"), "fetch_kib": 680141.8, "fetch_factor": 1.99995,
+               "write_kib": 270122.4, "write_factor": 1.0}
 
 
 def hmean(x):
