@@ -638,6 +638,8 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
     const uint64_t *F = xb_f(a, L);
     uint64_t *F1 = xb_f(a, L + 1);
     uint64_t *F2 = xb_f(a, L + 2);
+    const uint32_t *F32 = reinterpret_cast<const uint32_t *>(F);
+    const uint32_t *F1_32 = reinterpret_cast<const uint32_t *>(F1);
     const int32_t L1 = L + 1, L2 = L + 2;
     const int ln = xb_lane();
     XbWaveLds *wl = lds + (threadIdx.x >> 6);
@@ -703,17 +705,18 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
             }
 #pragma unroll
             for (int t = 0; t < 4; t++) h[t] = vc[t] >= 0 ? a.hd[vc[t]] : make_int2(-1, 0);
-            uint64_t fw[4], f1w[4];
+            // 32-bit probes: the dword of the bitmap word that holds the bit
+            uint32_t fw[4], f1w[4];
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                fw[t] = h[t].x >= 0 ? F[h[t].x >> 6] : 0ull;
-                f1w[t] = (h[t].x >= 0 && h[t].y == 1) ? F1[h[t].x >> 6] : 0ull;
+                fw[t] = h[t].x >= 0 ? F32[h[t].x >> 5] : 0u;
+                f1w[t] = (h[t].x >= 0 && h[t].y == 1) ? F1_32[h[t].x >> 5] : 0u;
             }
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                hit[t] = (fw[t] >> (h[t].x & 63)) & 1ull;
+                hit[t] = (fw[t] >> (h[t].x & 31)) & 1u;
                 done[t] = hit[t] || h[t].y == 1;
-                const bool p2 = !hit[t] && h[t].y == 1 && ((f1w[t] >> (h[t].x & 63)) & 1ull);
+                const bool p2 = !hit[t] && h[t].y == 1 && ((f1w[t] >> (h[t].x & 31)) & 1u);
                 if (hit[t] || p2) {
                     const int32_t v = vc[t];
                     const unsigned long long bit = 1ull << (v & 63);
@@ -778,10 +781,10 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                 const int32_t u1 = j + 1 < ce ? a.adj_bu[j + 1] : -1;
                 const int32_t u2 = j + 2 < ce ? a.adj_bu[j + 2] : -1;
                 const int32_t u3 = j + 3 < ce ? a.adj_bu[j + 3] : -1;
-                const bool h0 = (F[u0 >> 6] >> (u0 & 63)) & 1ull;
-                const bool h1 = u1 >= 0 && ((F[u1 >> 6] >> (u1 & 63)) & 1ull);
-                const bool h2 = u2 >= 0 && ((F[u2 >> 6] >> (u2 & 63)) & 1ull);
-                const bool h3 = u3 >= 0 && ((F[u3 >> 6] >> (u3 & 63)) & 1ull);
+                const bool h0 = (F32[u0 >> 5] >> (u0 & 31)) & 1u;
+                const bool h1 = u1 >= 0 && ((F32[u1 >> 5] >> (u1 & 31)) & 1u);
+                const bool h2 = u2 >= 0 && ((F32[u2 >> 5] >> (u2 & 31)) & 1u);
+                const bool h3 = u3 >= 0 && ((F32[u3 >> 5] >> (u3 & 31)) & 1u);
                 const unsigned long long bit = 1ull << (v & 63);
                 const int li = (int)((v >> 6) - w0);
                 if (h0 | h1 | h2 | h3) {
@@ -794,10 +797,10 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                     v = -1;
                 } else {
                     if (seen < 0) {
-                        if ((F1[u0 >> 6] >> (u0 & 63)) & 1ull) seen = u0;
-                        else if (u1 >= 0 && ((F1[u1 >> 6] >> (u1 & 63)) & 1ull)) seen = u1;
-                        else if (u2 >= 0 && ((F1[u2 >> 6] >> (u2 & 63)) & 1ull)) seen = u2;
-                        else if (u3 >= 0 && ((F1[u3 >> 6] >> (u3 & 63)) & 1ull)) seen = u3;
+                        if ((F1_32[u0 >> 5] >> (u0 & 31)) & 1u) seen = u0;
+                        else if (u1 >= 0 && ((F1_32[u1 >> 5] >> (u1 & 31)) & 1u)) seen = u1;
+                        else if (u2 >= 0 && ((F1_32[u2 >> 5] >> (u2 & 31)) & 1u)) seen = u2;
+                        else if (u3 >= 0 && ((F1_32[u3 >> 5] >> (u3 & 31)) & 1u)) seen = u3;
                     }
                     j += 4;
                     if (j >= ce) {
@@ -840,7 +843,7 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                 }
                 bool h[4];
 #pragma unroll
-                for (int t = 0; t < 4; t++) h[t] = u[t] >= 0 && ((F[u[t] >> 6] >> (u[t] & 63)) & 1ull);
+                for (int t = 0; t < 4; t++) h[t] = u[t] >= 0 && ((F32[u[t] >> 5] >> (u[t] & 31)) & 1u);
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
                     const unsigned long long m = __ballot(h[t]);
