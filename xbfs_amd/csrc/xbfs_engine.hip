@@ -51,10 +51,15 @@ __device__ __forceinline__ void xb_st(T *p, T v)
 }
 
 // Adds the block's totals of K counters to column `bid` of the partial
-// statistics (thread 0, plain read-modify-write: within a phase only this
-// block writes column bid).  Every thread of the block must call it.
+// statistics (thread 0, plain stores: within a phase only this block writes
+// column bid).  With `known`, old[k] is the column's current value, which
+// the caller tracks, and the store needs no load first; otherwise thread 0
+// reads the value back (one memory round trip before the level's barrier).
+// tot[k] receives the block totals (thread 0 only).  Every thread of the
+// block must call it.
 template <int K>
-__device__ void xb_part_add(const XbArgs &a, int bid, unsigned long long (&v)[K], const int (&f)[K])
+__device__ void xb_part_add(const XbArgs &a, int bid, unsigned long long (&v)[K], const int (&f)[K],
+                            bool known, const long long (&old)[K], long long (&tot)[K])
 {
     __shared__ unsigned long long sh[K][16];
     const int w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
@@ -69,10 +74,20 @@ __device__ void xb_part_add(const XbArgs &a, int bid, unsigned long long (&v)[K]
         for (int k = 0; k < K; k++) {
             unsigned long long t = 0;
             for (int i = 0; i < nw; i++) t += sh[k][i];
-            if (t) a.part[(int64_t)f[k] * XB_PMAX + bid] += (long long)t;
+            tot[k] = (long long)t;
+            if (t) {
+                long long *col = &a.part[(int64_t)f[k] * XB_PMAX + bid];
+                *col = (known ? old[k] : *col) + (long long)t;
+            }
         }
     }
     __syncthreads();
+}
+template <int K>
+__device__ void xb_part_add(const XbArgs &a, int bid, unsigned long long (&v)[K], const int (&f)[K])
+{
+    long long old[K] = {}, tot[K];
+    xb_part_add<K>(a, bid, v, f, false, old, tot);
 }
 
 // Sums of K fields over the first ncols columns of the partial statistics
@@ -357,7 +372,7 @@ __device__ __forceinline__ int xb_chunk_list(uint64_t myword, int64_t w0, XbWave
 // write; the second pass re-reads F_L and the degrees (mostly from L2).
 // With stats, adds (count, degree sum) to sweep slot L&1.
 __device__ void ph_sweep(const XbArgs &a, int32_t L, int bid, int nb, bool from_levels, bool stats,
-                         XbWaveLds *lds)
+                         XbWaveLds *lds, bool fresh = false)
 {
     __shared__ int s_wc[16][XB_BINS];
     __shared__ int s_base[XB_BINS];
@@ -481,9 +496,12 @@ __device__ void ph_sweep(const XbArgs &a, int32_t L, int bid, int nb, bool from_
     }
     }
     if (stats) {
+        // fresh: the sweep slot of level L was zeroed at level L-1 and only
+        // this sweep writes it, so the stores need not read it first
         unsigned long long vv[2] = {my_n, my_m};
         const int ff[2] = {xb_f_sweep(L, 0), xb_f_sweep(L, 1)};
-        xb_part_add<2>(a, bid, vv, ff);
+        long long old[2] = {0, 0}, tot[2];
+        xb_part_add<2>(a, bid, vv, ff, fresh, old, tot);
     }
 }
 
@@ -537,8 +555,14 @@ __device__ __forceinline__ void td_visit(const XbArgs &a, const XbTdCtx &t, int3
     }
 }
 
+// carry (persistent kernel, LDS; NULL elsewhere): what this block stored in
+// the (count, degree sum) fields of ring slot L+1 at level L-1 (its +2
+// discoveries), so the counts of level L are stored without reading the
+// column back; set to zero here, since a top-down level makes no +2
+// discoveries.
 template <bool SF>
-__device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue, XbSmem &sm)
+__device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue, XbSmem &sm,
+                      long long *carry = nullptr)
 {
     long long *s_pref = sm.td.pref;
     int32_t *s_u = sm.td.u;
@@ -626,14 +650,22 @@ __device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue,
     if (SF) {
         unsigned long long vv[2] = {my_n, my_m};
         const int ff[2] = {xb_f_ring(L + 1, 0), xb_f_ring(L + 1, 1)};
-        xb_part_add<2>(a, bid, vv, ff);
+        long long old[2] = {carry ? carry[0] : 0, carry ? carry[1] : 0}, tot[2];
+        xb_part_add<2>(a, bid, vv, ff, carry != nullptr, old, tot);
+    }
+    if (carry && threadIdx.x == 0) {
+        carry[0] = 0;
+        carry[1] = 0;
     }
 }
 
 // ---------------------------------------------------------------------------
 // bottom-up
 // ---------------------------------------------------------------------------
-__device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *lds)
+// carry: as for ph_td; afterwards it holds this block's +2 totals, stored in
+// ring slot L+2.
+__device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *lds,
+                      long long *carry = nullptr)
 {
     const uint64_t *F = xb_f(a, L);
     uint64_t *F1 = xb_f(a, L + 1);
@@ -885,7 +917,13 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
     unsigned long long vv[5] = {n1, m1, n2, m2, n2};
     const int ff[5] = {xb_f_ring(L1, 0), xb_f_ring(L1, 1), xb_f_ring(L2, 0), xb_f_ring(L2, 1),
                        xb_f_ring(L2, 2)};
-    xb_part_add<5>(a, bid, vv, ff);
+    // ring slot L+2 was zeroed at level L-1 and nothing wrote it since
+    long long old[5] = {carry ? carry[0] : 0, carry ? carry[1] : 0, 0, 0, 0}, tot[5];
+    xb_part_add<5>(a, bid, vv, ff, carry != nullptr, old, tot);
+    if (carry && threadIdx.x == 0) {
+        carry[0] = tot[2];
+        carry[1] = tot[3];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -957,6 +995,7 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
     __shared__ XbSmem sm;
     __shared__ XbLoopShared ls;
     __shared__ float s_mlp[XB_MLP_NPARAM]; // learned policy weights, loaded once (transposed)
+    __shared__ long long s_carry[2];       // see ph_td / ph_bu
     extern __shared__ __align__(16) unsigned char xb_dyn_lds[];
     XbWaveLds *lds = reinterpret_cast<XbWaveLds *>(xb_dyn_lds);
     if (a.mlp)
@@ -968,6 +1007,8 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
         if (!xb_grid_sync(a.ctl)) return;
     }
     if (threadIdx.x == 0) {
+        s_carry[0] = 0;
+        s_carry[1] = 0;
         xb_loop_init(&ls.st);
         if (a.nsrc > 0) ls.st.q_valid = 0; // the level-0 queue comes from a sweep of F_0
     }
@@ -982,7 +1023,7 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
         const unsigned long long t0 = wall_clock64();
         const bool need_sweep = ls.st.need_sweep != 0;
         if (need_sweep) {
-            ph_sweep(a, L, bid, nact, true, true, lds);
+            ph_sweep(a, L, bid, nact, true, true, lds, true);
             if (!xb_level_sync(c, nact)) return;
         }
         // Level statistics and the decision, by wave 0 alone, while the
@@ -1056,7 +1097,11 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
         } else if (nact == nb && ls.small) {
             if (bid >= XB_SMALL_BLOCKS) {
                 if (!xb_park(c)) return;
-                if (threadIdx.x == 0) ls = *a.lsg;
+                if (threadIdx.x == 0) {
+                    ls = *a.lsg;
+                    s_carry[0] = 0; // levels entered small mode without +2 discoveries
+                    s_carry[1] = 0;
+                }
                 __syncthreads();
                 if (!ls.open) return;
             } else {
@@ -1071,9 +1116,9 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
             ph_sweep(a, Lc, bid, nact, false, false, lds);
             if (!xb_level_sync(c, nact)) return;
         }
-        if (mode == XBFS_MODE_TD_SF) ph_td<true>(a, Lc, bid, nact, ls.p2_next == 0, sm);
-        else if (mode == XBFS_MODE_TD_SS) ph_td<false>(a, Lc, bid, nact, false, sm);
-        else ph_bu(a, Lc, bid, nact, lds);
+        if (mode == XBFS_MODE_TD_SF) ph_td<true>(a, Lc, bid, nact, ls.p2_next == 0, sm, s_carry);
+        else if (mode == XBFS_MODE_TD_SS) ph_td<false>(a, Lc, bid, nact, false, sm, s_carry);
+        else ph_bu(a, Lc, bid, nact, lds, s_carry);
         if (!xb_level_sync(c, nact)) return;
         if (threadIdx.x == 0) {
             if (bid == 0) {
