@@ -529,7 +529,7 @@ __device__ __forceinline__ void td_visit(const XbArgs &a, const XbTdCtx &t, int3
         if (SF) {
             won = atomicCAS(&a.level[v], -1, t.L1) == -1;
         } else {
-            a.level[v] = t.L1;
+            __builtin_nontemporal_store(t.L1, &a.level[v]);
             won = true; // every writer stores the same level; any parent is valid
         }
         if (won) {
@@ -664,6 +664,9 @@ __device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue,
 // ---------------------------------------------------------------------------
 // carry: as for ph_td; afterwards it holds this block's +2 totals, stored in
 // ring slot L+2.
+// Level (and top-down single-scan level) stores are nontemporal: each entry is
+// written once per traversal and not read back by its writer, so it need not
+// sit dirty in L2 until the level's release writes it back.
 __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *lds,
                       long long *carry = nullptr)
 {
@@ -753,7 +756,7 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                     const int32_t v = vc[t];
                     const unsigned long long bit = 1ull << (v & 63);
                     const int li = (int)((v >> 6) - w0);
-                    a.level[v] = hit[t] ? L1 : L2;
+                    __builtin_nontemporal_store(hit[t] ? L1 : L2, &a.level[v]);
                     if (a.parent) a.parent[v] = h[t].x;
                     atomicOr(hit[t] ? &wl->f1[li] : &wl->f2[li], bit);
                     atomicOr(&wl->va[li], bit);
@@ -820,7 +823,7 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                 const unsigned long long bit = 1ull << (v & 63);
                 const int li = (int)((v >> 6) - w0);
                 if (h0 | h1 | h2 | h3) {
-                    a.level[v] = L1;
+                    __builtin_nontemporal_store(L1, &a.level[v]);
                     if (a.parent) a.parent[v] = h0 ? u0 : (h1 ? u1 : (h2 ? u2 : u3));
                     atomicOr(&wl->f1[li], bit);
                     atomicOr(&wl->va[li], bit);
@@ -839,7 +842,7 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                         if (ce < e) {
                             defer = true; // unscanned edges remain: no +2 decision yet
                         } else if (seen >= 0) {
-                            a.level[v] = L2;
+                            __builtin_nontemporal_store(L2, &a.level[v]);
                             if (a.parent) a.parent[v] = seen;
                             atomicOr(&wl->f2[li], bit);
                             atomicOr(&wl->va[li], bit);
@@ -889,7 +892,7 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
             if (found >= 0 && ln == 0) {
                 const unsigned long long bit = 1ull << (v & 63);
                 const int li = (int)((v >> 6) - w0);
-                a.level[v] = L1;
+                __builtin_nontemporal_store(L1, &a.level[v]);
                 if (a.parent) a.parent[v] = found;
                 atomicOr(&wl->f1[li], bit);
                 atomicOr(&wl->va[li], bit);
