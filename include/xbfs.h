@@ -149,6 +149,9 @@ typedef struct {
     int32_t schedule_len;
     int32_t time_levels;    /* 1 = per-level device timing in rec[].t_s              */
     int32_t bin_block_max;  /* block bin: <= this, grid bin above; 0 = default       */
+    int32_t max_levels;     /* > 0: stop with XBFS_ERR_LIMIT at this many levels     */
+    int32_t wait_limit_us;  /* > 0: bound of one grid-barrier wait in the persistent
+                               kernel (default 4 s); past it XBFS_ERR_DEVICE          */
 } xbfs_opts;
 
 void xbfs_opts_default(xbfs_opts *o);

@@ -8,7 +8,7 @@ import pytest
 
 from oracle import oracle as O
 from tests import graphs as G
-from xbfs_amd import (Context, Graph, MODE_BU, MODE_TD_SF, MODE_TD_SS, POLICY_FORCE_BU,
+from xbfs_amd import (Context, Graph, XbfsError, MODE_BU, MODE_TD_SF, MODE_TD_SS, POLICY_FORCE_BU,
                       POLICY_FORCE_TD_SF, POLICY_FORCE_TD_SS, POLICY_HEURISTIC,
                       POLICY_SCHEDULE, default_opts)
 
@@ -111,6 +111,40 @@ def test_csr_row_order(monkeypatch):
         for pol in POLICIES[:4]:
             say("csr order rmat14", s, pol)
             run_check(ctx, g, s, **pol)
+    ctx.close()
+
+
+def test_level_cap_then_full_run():
+    """opts.max_levels, forced from the host, ends a 299-level traversal with
+    XBFS_ERR_LIMIT under both drivers; the same context then runs complete
+    traversals correctly (the failure leaves nothing behind)."""
+    g = Graph(*G.path(300))
+    ctx = Context(g)
+    for eng in (1, 0):
+        for cap in (1, 10, 298):
+            say("level cap", eng, cap)
+            with pytest.raises(XbfsError) as ei:
+                ctx.bfs(0, persistent=eng, max_levels=cap)
+            assert ei.value.code == 7  # XBFS_ERR_LIMIT
+            run_check(ctx, g, 0, engines=(eng,))
+    run_check(ctx, g, 0, max_levels=299)  # exactly enough levels
+    ctx.close()
+
+
+def test_barrier_wait_limit_then_full_run():
+    """A 1 us bound on each grid-barrier wait of the persistent kernel (the
+    kernel's own bounded wait, tightened from the host) ends the traversal
+    with XBFS_ERR_DEVICE: blocks that time out set the error word, every other
+    block leaves at its next barrier, so the kernel still exits normally.  The
+    same context then runs complete traversals correctly."""
+    g = Graph(*G.grid(64, 64))
+    ctx = Context(g)
+    for pol in (dict(policy=POLICY_HEURISTIC), dict(policy=POLICY_FORCE_BU)):
+        say("wait limit", pol)
+        with pytest.raises(XbfsError) as ei:
+            ctx.bfs(0, persistent=1, wait_limit_us=1, **pol)
+        assert ei.value.code == 8  # XBFS_ERR_DEVICE
+        run_check(ctx, g, 0, engines=(1,), **pol)
     ctx.close()
 
 

@@ -72,6 +72,15 @@ int xb_ctx_free_device(xbfs_ctx *c)
 
 extern "C" {
 
+// Bound of one grid-barrier wait in the persistent kernel, in wall-clock
+// ticks: opts.wait_limit_us, or 4 s.
+static unsigned long long wait_ticks(const xbfs_ctx *c, const xbfs_opts &o)
+{
+    const double us = o.wait_limit_us > 0 ? (double)o.wait_limit_us : 4.0e6;
+    const double t = us * 1e-6 * c->wall_hz;
+    return t < 1.0 ? 1ull : (unsigned long long)t;
+}
+
 void xbfs_opts_default(xbfs_opts *o)
 {
     if (!o) return;
@@ -90,6 +99,8 @@ void xbfs_opts_default(xbfs_opts *o)
     o->schedule_len = 0;
     o->time_levels = 0;
     o->bin_block_max = XB_DEFAULT_T3;
+    o->max_levels = 0;
+    o->wait_limit_us = 0;
 }
 
 int64_t xbfs_num_vertices(const xbfs_ctx *c) { return c ? c->n : -1; }
@@ -273,6 +284,7 @@ int xb_resolve_opts(const xbfs_opts *in, xbfs_opts *o)
             if (o->schedule[i] < 0 || o->schedule[i] > 2) return XBFS_ERR_ARG;
     }
     if (!(o->alpha >= 0.0f)) return XBFS_ERR_ARG;
+    if (o->max_levels < 0 || o->wait_limit_us < 0) return XBFS_ERR_ARG;
     return XBFS_OK;
 }
 
@@ -323,6 +335,7 @@ static XbArgs make_args(xbfs_ctx *c, int32_t src, const xbfs_opts &o)
     a.policy = o.policy;
     a.alpha = o.alpha;
     a.max_levels = (int32_t)std::min<int64_t>(c->n + 1, INT32_MAX - 4);
+    if (o.max_levels > 0) a.max_levels = std::min(a.max_levels, o.max_levels);
     return a;
 }
 
@@ -469,7 +482,7 @@ static int bfs_persistent(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs
 {
     int grid = c->coop_blocks;
     if (o.grid_dim > 0 && o.grid_dim < grid) grid = o.grid_dim;
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl);
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl, wait_ticks(c, o));
     XB_LAUNCHED();
     XB_HIP(hipEventRecord(c->ev_start, c->stream));
     XbArgs args = a;
@@ -1007,7 +1020,7 @@ extern "C" int xbfs_bfs_probe(xbfs_ctx *c, int32_t src, const xbfs_opts *opts, i
                               hipMemcpyHostToDevice, c->stream));
     const XbArgs a = make_args(c, src, o);
     const int grid = o.grid_dim > 0 ? o.grid_dim : c->num_cu * 8;
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl);
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl, wait_ticks(c, o));
     hipLaunchKernelGGL(k_ph_init, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a);
     if (a.nsrc > 0) hipLaunchKernelGGL(k_ph_seed, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a);
     XB_LAUNCHED();

@@ -34,7 +34,6 @@
 #include "xbfs_engine.h"
 #include "xbfs_policy.h"
 
-#define XB_BAR_TIMEOUT_TICKS (100ull * 1000 * 1000 * 4) /* ~4 s at the 100 MHz wall clock */
 
 __device__ __forceinline__ unsigned long long xb_ld_u64(const unsigned long long *p)
 {
@@ -124,9 +123,9 @@ __device__ void xb_part_sum(const XbArgs &a, const int (&f)[K], long long (&out)
 
 // Barrier among the first `members` blocks of a cooperative launch (all
 // blocks co-resident) on the counter / generation pair (cnt, gen).  The wait
-// is bounded: after ~4 s without progress, or when another block reported a
-// timeout, it sets ctl->error and returns false so every block leaves the
-// kernel.  With watch_hb, progress also means a change of ctl->heartbeat
+// is bounded: after ctl->wait_ticks without progress (4 s unless the caller
+// asked for less), or when another block reported a timeout, it sets
+// ctl->error and returns false so every block leaves the kernel.  With watch_hb, progress also means a change of ctl->heartbeat
 // (blocks parked during small-frontier levels wait as long as those levels
 // keep coming).
 __device__ bool xb_barrier(XbCtl *c, uint32_t *cnt, uint32_t *genw, uint32_t members, bool watch_hb)
@@ -148,6 +147,8 @@ __device__ bool xb_barrier(XbCtl *c, uint32_t *cnt, uint32_t *genw, uint32_t mem
             __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(genw, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
+            const unsigned long long limit =
+                __hip_atomic_load(&c->wait_ticks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             unsigned long long t0 = wall_clock64();
             uint32_t hb = watch_hb ? __hip_atomic_load(&c->heartbeat, __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_AGENT)
@@ -162,9 +163,18 @@ __device__ bool xb_barrier(XbCtl *c, uint32_t *cnt, uint32_t *genw, uint32_t mem
                         t0 = wall_clock64();
                     }
                 }
-                if (__hip_atomic_load(&c->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-                    wall_clock64() - t0 > XB_BAR_TIMEOUT_TICKS) {
-                    __hip_atomic_store(&c->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                // another block's error (a timeout, or the level cap that
+                // block 0 reports before the last barrier) ends the wait and
+                // keeps its code; a timeout of this wait reports 1 unless an
+                // error is already there
+                if (__hip_atomic_load(&c->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    ok = 0;
+                    break;
+                }
+                if (wall_clock64() - t0 > limit) {
+                    int32_t none = 0;
+                    __hip_atomic_compare_exchange_strong(&c->error, &none, 1, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     ok = 0;
                     break;
                 }
@@ -1234,9 +1244,10 @@ __global__ void __launch_bounds__(1024) k_part_reduce(XbArgs a)
     }
 }
 
-__global__ void k_ctl_reset(XbCtl *c)
+__global__ void k_ctl_reset(XbCtl *c, unsigned long long wait_ticks)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
+        xb_st(&c->wait_ticks, wait_ticks);
         xb_st(&c->bar_count, 0u);
         xb_st(&c->bar_gen, 0u);
         xb_st(&c->sbar_count, 0u);
