@@ -148,6 +148,40 @@ def test_barrier_wait_limit_then_full_run():
     ctx.close()
 
 
+def test_hub_component_termination(monkeypatch):
+    """A traversal from the largest-degree vertex's component ends as soon as
+    every member has its level, one level (or, after +2 discoveries, two)
+    sooner than waiting for an empty frontier, and its bottom-up candidates
+    leave out the other components.  Sources outside the component run as
+    before.  Levels, parents and reach equal the oracle's either way, and
+    equal a context built with XBFS_COMPONENT=off."""
+    beg, adj = G.disjoint(G.star(100), G.path(40), G.grid(9, 9), G.csr_from_edges(2, [(0, 1)]),
+                          G.csr_from_edges(5, []))
+    g = Graph(beg, adj)
+    ctx = Context(g)
+    monkeypatch.setenv("XBFS_COMPONENT", "off")
+    plain = Context(g)
+    for src in (0, 1, 50, 99, 100, 139, 140, 200, 221, 222):
+        for pol in POLICIES:
+            say("hub component", src, pol)
+            for eng in (1, 0):
+                lv, _, st = _check_one(ctx, g, src, persistent=eng, **pol)
+                lv0, _, st0 = _check_one(plain, g, src, persistent=eng, **pol)
+                assert (lv == lv0).all()
+                if src < 100:
+                    assert st["n_levels"] < st0["n_levels"], (st, st0)
+                else:
+                    assert st["n_levels"] == st0["n_levels"], (st, st0)
+    # multi-source traversals keep the empty-frontier end (seeds may lie in
+    # several components)
+    for srcs in ([0], [5, 120], [120, 5, 222]):
+        lv, _, _, st = ctx.bfs_multi(np.array(srcs, np.int32))
+        want = O.bfs_multi(g.beg_pos, g.adj, np.array(srcs, np.int32))[0]
+        assert (lv == want).all()
+    ctx.close()
+    plain.close()
+
+
 def test_degree_zero_source():
     beg, adj = G.disjoint(G.path(10), G.csr_from_edges(3, []))
     g = Graph(beg, adj)

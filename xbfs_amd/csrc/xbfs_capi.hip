@@ -51,7 +51,7 @@ bool pow2_block(int b) { return b >= 64 && b <= 1024 && (b & (b - 1)) == 0; }
 int xb_ctx_free_device(xbfs_ctx *c)
 {
     (void)hipSetDevice(c->device);
-    void *bufs[] = {c->d_beg,   c->d_adj,  c->d_adj_bu, c->d_nz,  c->d_hd,   c->d_vis,
+    void *bufs[] = {c->d_beg,   c->d_adj,  c->d_adj_bu, c->d_nz,  c->d_comp, c->d_hd,   c->d_vis,
                     c->d_fbm,   c->d_level, c->d_parent, c->d_qbuf, c->d_ctl,  c->d_lsg,
                     c->d_recs,  c->d_sched, c->d_mlp,
                     c->d_list,  c->d_ubits, c->d_tile, c->d_list_cnt, c->s_level,
@@ -155,6 +155,8 @@ static bool sort_rows_by_degree(xbfs_ctx *c, int64_t max_deg)
     return ok;
 }
 
+static int find_hub_component(xbfs_ctx *c, int32_t hub);
+
 int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, int64_t m,
                 xbfs_ctx **out)
 {
@@ -175,10 +177,13 @@ int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, i
     c->ntiles_bu = ceil_div(c->nwords, XB_BU_TILE_WORDS);
     c->fstride = c->ntiles_bu * XB_BU_TILE_WORDS;
     c->ntiles_list = ceil_div(c->nwords, XB_TILE_WORDS);
-    int64_t nz = 0, max_deg = 0;
+    int64_t nz = 0, max_deg = 0, hub = -1;
     for (int64_t i = 0; i < n; i++) {
         nz += beg[i + 1] > beg[i];
-        max_deg = std::max<int64_t>(max_deg, beg[i + 1] - beg[i]);
+        if (beg[i + 1] - beg[i] > max_deg) {
+            max_deg = beg[i + 1] - beg[i];
+            hub = i;
+        }
     }
     c->n_nz = nz;
     auto fail = [&](int code) {
@@ -250,6 +255,11 @@ int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, i
     }
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
         return fail(XBFS_ERR_HIP);
+    // XBFS_COMPONENT=off: every traversal runs until its frontier is empty
+    const char *comp = getenv("XBFS_COMPONENT");
+    if (hub >= 0 && !(comp && strcmp(comp, "off") == 0)) {
+        if ((rc = find_hub_component(c, (int32_t)hub)) != XBFS_OK) return fail(rc);
+    }
     *out = c;
     return XBFS_OK;
 }
@@ -325,6 +335,12 @@ static XbArgs make_args(xbfs_ctx *c, int32_t src, const xbfs_opts &o)
     a.ntiles = c->ntiles_bu;
     a.fstride = c->fstride;
     a.n_nz = c->n_nz;
+    a.n_target = c->n_nz;
+    if (c->d_comp && src >= 0 && (int64_t)src < c->n &&
+        ((c->h_comp[(size_t)src >> 6] >> (src & 63)) & 1ull)) {
+        a.nz = c->d_comp;
+        a.n_target = c->n_comp;
+    }
     a.srcs = nullptr;
     a.nsrc = 0;
     a.hub_first = c->hub_first;
@@ -561,6 +577,8 @@ static int bfs_run(xbfs_ctx *c, int32_t src, const int32_t *srcs, int32_t nsrc,
         a.srcs = c->d_srcs;
         a.nsrc = nsrc;
         a.src = srcs[0];
+        a.nz = c->d_nz; // seeds may lie in several components
+        a.n_target = c->n_nz;
     }
     rc = o.persistent ? bfs_persistent(c, a, o, rec, rec_cap, st)
                       : bfs_host_loop(c, a, o, rec, rec_cap, st);
@@ -583,6 +601,41 @@ int xbfs_bfs(xbfs_ctx *c, int32_t src, const xbfs_opts *opts, int32_t *level_out
     if (src < 0 || (int64_t)src >= c->n) return XBFS_ERR_ARG;
     return bfs_run(c, src, nullptr, 0, opts, level_out, parent_out, rec, rec_cap, st);
 }
+
+} // extern "C"
+
+// One traversal from the largest-degree vertex marks its component (the giant
+// component of a skewed graph).  When the component leaves out some
+// non-isolated vertex, a later traversal from a member ends as soon as every
+// member has its level instead of expanding its last level to find nothing,
+// and its bottom-up candidates skip the vertices outside (no path reaches
+// them).  When memory for the bitmap is not available every traversal runs
+// until its frontier is empty, as without the component.
+static int find_hub_component(xbfs_ctx *c, int32_t hub)
+{
+    xbfs_opts o;
+    xbfs_opts_default(&o);
+    xbfs_stats st;
+    int rc = bfs_run(c, hub, nullptr, 0, &o, nullptr, nullptr, nullptr, 0, &st);
+    if (rc) return rc;
+    if (st.v_cc >= c->n_nz) return XBFS_OK; // nothing to leave out
+    if (hipMalloc(&c->d_comp, sizeof(uint64_t) * (size_t)c->nwords) != hipSuccess) {
+        (void)hipGetLastError();
+        c->d_comp = nullptr;
+        return XBFS_OK;
+    }
+    hipLaunchKernelGGL(k_reached_bits, dim3(grid_for(c->nwords * 64, 256, 65536)), dim3(256), 0,
+                       c->stream, c->d_level, c->n, c->d_comp, c->nwords);
+    XB_LAUNCHED();
+    c->h_comp.resize((size_t)c->nwords);
+    XB_HIP(hipMemcpyAsync(c->h_comp.data(), c->d_comp, sizeof(uint64_t) * (size_t)c->nwords,
+                          hipMemcpyDeviceToHost, c->stream));
+    XB_HIP(hipStreamSynchronize(c->stream));
+    c->n_comp = st.v_cc;
+    return XBFS_OK;
+}
+
+extern "C" {
 
 int xbfs_bfs_multi(xbfs_ctx *c, const int32_t *srcs, int32_t nsrc, const xbfs_opts *opts,
                    int32_t *level_out, int32_t *parent_out, xbfs_level_rec *rec, int32_t rec_cap,

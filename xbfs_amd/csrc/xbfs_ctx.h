@@ -25,6 +25,14 @@ struct xbfs_ctx {
     int64_t *d_beg = nullptr;   // int64[n+1]
     int32_t *d_adj = nullptr;   // int32[m]
     uint64_t *d_nz = nullptr;   // bit v: deg(v) > 0
+    // The component of the largest-degree vertex (the giant component of a
+    // skewed graph), found at create by one traversal from that vertex: a
+    // traversal from a member ends once n_comp vertices have levels, and its
+    // bottom-up candidates leave out the vertices no path reaches.  NULL when
+    // the component holds every non-isolated vertex (nothing to leave out).
+    uint64_t *d_comp = nullptr;
+    std::vector<uint64_t> h_comp; // host copy (membership of a source)
+    int64_t n_comp = 0;
     int2 *d_hd = nullptr;       // {largest-degree neighbour, degree} per vertex
     int32_t *d_adj_bu = nullptr; // bottom-up copy of d_adj, rows by neighbour degree (or NULL)
     int32_t hub_first = 0;      // d_adj_bu is built (each row starts with its hub)

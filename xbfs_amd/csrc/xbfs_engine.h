@@ -73,7 +73,7 @@ struct XbArgs {
     const int64_t *beg;
     const int32_t *adj;
     const int32_t *adj_bu; // rows by neighbour degree, hub first (hub_first), else adj
-    const uint64_t *nz; // bit v: deg(v) > 0
+    const uint64_t *nz; // bottom-up candidates: bit v: deg(v) > 0, or v in the source's component
     const int2 *hd;     // {largest-degree neighbour, degree} per vertex
     uint64_t *vis;      // bit v: level[v] != -1
     uint64_t *fbm;      // XB_FRINGS bitmaps of nwords: bit v of ring l&3 <=> level[v] == l
@@ -87,6 +87,8 @@ struct XbArgs {
     int32_t sched_len;
     const float *mlp;   // learned policy weights (NULL = none)
     int64_t n, m, nwords, ntiles, n_nz;
+    int64_t n_target; // the traversal is complete once this many vertices have levels:
+                      // n_nz, or the size of the source's component when it is known
     int64_t fstride; // words between frontier rings: nwords rounded up to a tile (256 B)
     const int32_t *srcs; // multi-source seeds (device), used when nsrc > 0
     int32_t nsrc;

@@ -943,15 +943,16 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
 // level loop (shared by both drivers through these helpers)
 // ---------------------------------------------------------------------------
 // Level bookkeeping shared by both drivers.  Returns false when the
-// traversal is complete (no frontier edges, or every non-isolated vertex
-// already has its level); otherwise fills the unvisited counts of level L.
+// traversal is complete (no frontier edges, or every vertex it can reach
+// already has its level: a.n_target); otherwise fills the unvisited counts of
+// level L, which count every non-isolated vertex as reachable.
 __host__ __device__ inline bool xb_level_open(XbLoop &st, const XbArgs &a, long long n_f,
                                               long long m_f, long long pend_n, long long pend_m,
                                               long long *n_u, long long *m_u)
 {
     st.cum_n += n_f;
     st.cum_m += m_f;
-    if (m_f == 0 || st.cum_n + pend_n >= a.n_nz) return false;
+    if (m_f == 0 || st.cum_n + pend_n >= a.n_target) return false;
     *n_u = a.n_nz - (st.cum_n + pend_n);
     *m_u = a.m - (st.cum_m + pend_m);
     return true;

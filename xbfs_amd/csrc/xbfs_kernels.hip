@@ -26,6 +26,19 @@ __global__ void __launch_bounds__(256) k_build_nz(const int64_t *__restrict__ be
     }
 }
 
+// bits[w] bit i <=> level[64 w + i] >= 0: the vertices a finished traversal
+// reached (its source's component)
+__global__ void __launch_bounds__(256) k_reached_bits(const int32_t *__restrict__ level, int64_t n,
+                                                      uint64_t *__restrict__ bits, int64_t nwords)
+{
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nwords; w += nwaves) {
+        const int64_t v = w * 64 + xb_lane();
+        const unsigned long long m = __ballot(v < n && level[v] >= 0);
+        if (xb_lane() == 0) bits[w] = m;
+    }
+}
+
 // Hub table: hd[v] = {the neighbour of v with the largest degree (first in
 // adjacency order on ties), deg(v)}; {-1, 0} for an isolated vertex.
 // Bottom-up probes the frontier bit of that neighbour before touching v's
