@@ -111,7 +111,8 @@ __host__ __device__ inline uint64_t *xb_f(const XbArgs &a, int32_t level)
 // Block-wide LDS shared by the phases (static).
 union XbSmem {
     struct {
-        long long pref[XB_GCAP + 1]; // exclusive degree prefix of grid-bin vertices
+        long long pref[XB_GCAP + 1]; // exclusive degree prefix of the vertices of one pass
+        long long b[XB_GCAP];        // their row starts
         int32_t u[XB_GCAP];
     } td;
     struct {

@@ -565,6 +565,54 @@ __device__ __forceinline__ void td_visit(const XbArgs &a, const XbTdCtx &t, int3
     }
 }
 
+// Visits the rows of cnt <= XB_GCAP frontier vertices sel(0..cnt-1) as one
+// concatenated adjacency shared by every thread of the active grid: each block
+// scans their degrees in LDS (with their row starts), and a thread maps an
+// edge position to its vertex by binary search.  Every thread of the block
+// calls it (block barriers inside).
+template <bool SF, typename Sel>
+__device__ void td_concat(const XbArgs &a, const XbTdCtx &t, int32_t cnt, Sel sel, XbSmem &sm,
+                          int64_t gtid, int64_t nthr, unsigned long long &my_n,
+                          unsigned long long &my_m)
+{
+    long long *s_pref = sm.td.pref, *s_b = sm.td.b;
+    int32_t *s_u = sm.td.u;
+    // two elements per thread (blockDim.x >= 256, so 2 * blockDim.x >= XB_GCAP)
+    const int k0 = 2 * threadIdx.x, k1 = k0 + 1;
+    long long d0 = 0, d1 = 0;
+    if (k0 < cnt) {
+        const int32_t u = sel(k0);
+        const int64_t b = a.beg[u];
+        s_u[k0] = u;
+        s_b[k0] = b;
+        d0 = a.beg[u + 1] - b;
+    }
+    if (k1 < cnt) {
+        const int32_t u = sel(k1);
+        const int64_t b = a.beg[u];
+        s_u[k1] = u;
+        s_b[k1] = b;
+        d1 = a.beg[u + 1] - b;
+    }
+    long long tot = 0;
+    const long long ex = xb_block_excl_scan_i64(d0 + d1, &tot);
+    if (k0 < cnt) s_pref[k0] = ex;
+    if (k1 < cnt) s_pref[k1] = ex + d0;
+    if (threadIdx.x == 0) s_pref[cnt] = tot;
+    __syncthreads();
+    const long long T = s_pref[cnt];
+    for (long long p = gtid; p < T; p += nthr) {
+        int lo = 0, hi = cnt - 1; // largest k with s_pref[k] <= p
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_pref[mid] <= p) lo = mid;
+            else hi = mid - 1;
+        }
+        td_visit<SF>(a, t, s_u[lo], a.adj[s_b[lo] + (p - s_pref[lo])], true, my_n, my_m);
+    }
+    __syncthreads();
+}
+
 // carry (persistent kernel, LDS; NULL elsewhere): what this block stored in
 // the (count, degree sum) fields of ring slot L+1 at level L-1 (its +2
 // discoveries), so the counts of level L are stored without reading the
@@ -574,8 +622,6 @@ template <bool SF>
 __device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue, XbSmem &sm,
                       long long *carry = nullptr)
 {
-    long long *s_pref = sm.td.pref;
-    int32_t *s_u = sm.td.u;
     XbTdCtx t;
     const int qc = L % XB_QBUFS, qn = (L + 1) % XB_QBUFS;
     for (int k = 0; k < XB_BINS; k++) {
@@ -592,70 +638,50 @@ __device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue,
     const int64_t nthr = (int64_t)nb * blockDim.x;
     const int ln = xb_lane();
 
-    // grid bin: all threads of the grid share the concatenated adjacency of
-    // the bin's vertices; each block scans their degrees in LDS and maps an
-    // edge position to its vertex by binary search.
-    for (int32_t base = 0; base < t.nc[3]; base += XB_GCAP) {
-        const int32_t cnt = min(XB_GCAP, t.nc[3] - base);
-        // two elements per thread (blockDim.x >= 256, so 2 * blockDim.x >= XB_GCAP)
-        const int k0 = 2 * threadIdx.x, k1 = k0 + 1;
-        long long d0 = 0, d1 = 0;
-        if (k0 < cnt) {
-            const int32_t u = t.qc[3][base + k0];
-            s_u[k0] = u;
-            d0 = a.beg[u + 1] - a.beg[u];
-        }
-        if (k1 < cnt) {
-            const int32_t u = t.qc[3][base + k1];
-            s_u[k1] = u;
-            d1 = a.beg[u + 1] - a.beg[u];
-        }
-        long long tot = 0;
-        const long long ex = xb_block_excl_scan_i64(d0 + d1, &tot);
-        if (k0 < cnt) s_pref[k0] = ex;
-        if (k1 < cnt) s_pref[k1] = ex + d0;
-        if (threadIdx.x == 0) s_pref[cnt] = tot;
-        __syncthreads();
-        const long long T = s_pref[cnt];
-        for (long long p = gtid; p < T; p += nthr) {
-            int lo = 0, hi = cnt - 1; // largest k with s_pref[k] <= p
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_pref[mid] <= p) lo = mid;
-                else hi = mid - 1;
-            }
-            const int32_t u = s_u[lo];
-            const int64_t j = a.beg[u] + (p - s_pref[lo]);
-            td_visit<SF>(a, t, u, a.adj[j], true, my_n, my_m);
-        }
-        __syncthreads();
+    // A frontier of at most XB_GCAP vertices, whatever their bins, is one
+    // concatenated adjacency for the whole grid (no lane or wave walks a row
+    // serially, so a level with a few hubs or a few dozen mid-degree vertices
+    // takes about one round of dependent loads); otherwise the grid bin is,
+    // XB_GCAP vertices per pass, and the other bins follow.
+    const int32_t nall = t.nc[0] + t.nc[1] + t.nc[2] + t.nc[3];
+    const bool flat = nall <= XB_GCAP;
+    const int32_t c0 = t.nc[0], c1 = c0 + t.nc[1], c2 = c1 + t.nc[2];
+    const int32_t ncat = flat ? nall : t.nc[3];
+    for (int32_t base = 0; base < ncat; base += XB_GCAP) {
+        td_concat<SF>(a, t, min(XB_GCAP, ncat - base), [&](int32_t k) {
+            if (!flat) return t.qc[3][base + k];
+            return k < c0 ? t.qc[0][k] : k < c1 ? t.qc[1][k - c0] : k < c2 ? t.qc[2][k - c1]
+                                                                            : t.qc[3][k - c2];
+        }, sm, gtid, nthr, my_n, my_m);
     }
-    // block bin: one block per vertex
-    for (int32_t i = bid; i < t.nc[2]; i += nb) {
-        const int32_t u = t.qc[2][i];
-        const int64_t b = a.beg[u], e = a.beg[u + 1];
-        for (int64_t j = b + threadIdx.x; j < e; j += blockDim.x)
-            td_visit<SF>(a, t, u, a.adj[j], true, my_n, my_m);
-    }
-    // Wave and thread bins: work is dealt to blocks round robin (wave-bin
-    // vertex i -> block i mod nb; thread-bin entries in runs of 64, run r ->
-    // block r mod nb, so a wave still takes 64 consecutive entries), so a
-    // frontier of a few thousand vertices spreads over every CU instead of
-    // filling the first few.
-    // wave bin: one wave per vertex
-    const int64_t gwi = (int64_t)(threadIdx.x >> 6) * nb + bid;
-    for (int64_t i = gwi; i < t.nc[1]; i += nthr >> 6) {
-        const int32_t u = t.qc[1][i];
-        const int64_t b = a.beg[u], e = a.beg[u + 1];
-        for (int64_t j = b + ln; j < e; j += XB_WAVE)
-            td_visit<SF>(a, t, u, a.adj[j], true, my_n, my_m);
-    }
-    // thread bin: one lane per vertex
-    const int64_t gti = (((int64_t)(threadIdx.x >> 6) * nb + bid) << 6) + ln;
-    for (int64_t i = gti; i < t.nc[0]; i += nthr) {
-        const int32_t u = t.qc[0][i];
-        const int64_t b = a.beg[u], e = a.beg[u + 1];
-        for (int64_t j = b; j < e; j++) td_visit<SF>(a, t, u, a.adj[j], true, my_n, my_m);
+    if (!flat) {
+        // block bin: one block per vertex
+        for (int32_t i = bid; i < t.nc[2]; i += nb) {
+            const int32_t u = t.qc[2][i];
+            const int64_t b = a.beg[u], e = a.beg[u + 1];
+            for (int64_t j = b + threadIdx.x; j < e; j += blockDim.x)
+                td_visit<SF>(a, t, u, a.adj[j], true, my_n, my_m);
+        }
+        // Wave and thread bins: work is dealt to blocks round robin (wave-bin
+        // vertex i -> block i mod nb; thread-bin entries in runs of 64, run r ->
+        // block r mod nb, so a wave still takes 64 consecutive entries), so a
+        // frontier of a few thousand vertices spreads over every CU instead of
+        // filling the first few.
+        // wave bin: one wave per vertex
+        const int64_t gwi = (int64_t)(threadIdx.x >> 6) * nb + bid;
+        for (int64_t i = gwi; i < t.nc[1]; i += nthr >> 6) {
+            const int32_t u = t.qc[1][i];
+            const int64_t b = a.beg[u], e = a.beg[u + 1];
+            for (int64_t j = b + ln; j < e; j += XB_WAVE)
+                td_visit<SF>(a, t, u, a.adj[j], true, my_n, my_m);
+        }
+        // thread bin: one lane per vertex
+        const int64_t gti = (((int64_t)(threadIdx.x >> 6) * nb + bid) << 6) + ln;
+        for (int64_t i = gti; i < t.nc[0]; i += nthr) {
+            const int32_t u = t.qc[0][i];
+            const int64_t b = a.beg[u], e = a.beg[u + 1];
+            for (int64_t j = b; j < e; j++) td_visit<SF>(a, t, u, a.adj[j], true, my_n, my_m);
+        }
     }
     if (SF) {
         unsigned long long vv[2] = {my_n, my_m};
