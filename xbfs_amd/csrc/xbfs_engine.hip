@@ -902,15 +902,15 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
         // scan at the first level-L neighbour.  The +2 rule is not applied to
         // them: one left unvisited is found at the next level.
         for (int d = 0; d < ndef; d++) {
-            const int32_t v = wl->list[d];
-            const int64_t b = a.beg[v], e = a.beg[v + 1];
+            const int32_t dv = wl->list[d];
+            const int64_t db = a.beg[dv], de = a.beg[dv + 1];
             int32_t found = -1;
-            for (int64_t base = b + skip + XB_BU_LANE_MAX; base < e; base += 256) {
+            for (int64_t base = db + skip + XB_BU_LANE_MAX; base < de; base += 256) {
                 int32_t u[4];
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
                     const int64_t jj = base + ln + 64 * t;
-                    u[t] = jj < e ? a.adj_bu[jj] : -1;
+                    u[t] = jj < de ? a.adj_bu[jj] : -1;
                 }
                 bool h[4];
 #pragma unroll
@@ -926,14 +926,14 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                 if (found >= 0) break;
             }
             if (found >= 0 && ln == 0) {
-                const unsigned long long bit = 1ull << (v & 63);
-                const int li = (int)((v >> 6) - w0);
-                __builtin_nontemporal_store(L1, &a.level[v]);
-                if (a.parent) a.parent[v] = found;
+                const unsigned long long bit = 1ull << (dv & 63);
+                const int li = (int)((dv >> 6) - w0);
+                __builtin_nontemporal_store(L1, &a.level[dv]);
+                if (a.parent) a.parent[dv] = found;
                 atomicOr(&wl->f1[li], bit);
                 atomicOr(&wl->va[li], bit);
                 n1 += 1;
-                m1 += (unsigned long long)(e - b);
+                m1 += (unsigned long long)(de - db);
             }
         }
         xb_wave_sync();
