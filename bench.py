@@ -50,11 +50,12 @@ HBM_PEAK = 8.0e12
 # HBM bytes per k_persistent launch on the default workload (RMAT-24 ef16, the
 # 64 seeded sources, learned policy), from two rocprofv3 --pmc passes
 # (profiles/k_persistent_rmat24_pmc_traffic.md): mean FETCH_SIZE and
-# WRITE_SIZE in KiB per dispatch, each scaled by what the same counter reads
+# WRITE_SIZE in KiB per dispatch over the 65 traversal launches (the context's
+# create-time traversal left out), each scaled by what the same counter reads
 # for a plain 1 GiB device copy (FETCH_SIZE reads half of the bytes a copy
 # moves: factor 2.0; WRITE_SIZE reads all of them: factor 1.0).
-PMC_TRAFFIC = {"workload": (24, 16, 64, "learned"), "fetch_kib": 680141.8, "fetch_factor": 1.99995,
-               "write_kib": 270122.4, "write_factor": 1.0}
+PMC_TRAFFIC = {"workload": (24, 16, 64, "learned"), "fetch_kib": 633672.2, "fetch_factor": 1.99995,
+               "write_kib": 269223.9, "write_factor": 1.0}
 
 
 def hmean(x):
