@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from tests import graphs as G
 from xbfs_amd import _capi as C
 from xbfs_amd import policy as P
 from xbfs_amd import Context, Graph, POLICY_HEURISTIC, POLICY_LEARNED
@@ -156,3 +157,37 @@ def test_trained_policy_gain_on_held_out_graph():
           f"{tot['learned']*1e3:.3f} ms, oracle-best {tot['oracle_best']*1e3:.3f} ms, "
           f"gain {gain:.3f}x", flush=True)
     assert gain >= 0.97
+
+
+@pytest.mark.gpu
+def test_learned_policy_domain_guard():
+    """The learned policy decides only inside the domain it was trained on:
+    skewed graphs (largest degree at least 16 times the mean degree of the
+    non-isolated vertices) and levels below 16.  On a grid and an
+    Erdos-Renyi graph, and on the deep levels of a skewed graph with a long
+    path, the heuristic decides (records carry policy 0, modes equal the
+    heuristic's); levels stay exact everywhere."""
+    rng = np.random.default_rng(4)
+    w = P.load() if os.path.exists(P.WEIGHTS) else \
+        (rng.standard_normal(C.MLP_NPARAM) * 0.5).astype(np.float32)
+    star_path = G.csr_from_edges(1040, [(0, i) for i in range(1, 1000)] +
+                                 [(999 + i, 1000 + i) for i in range(40)])
+    cases = {"grid": G.grid(64, 64), "er": None, "star_path": star_path}
+    for name, csr in cases.items():
+        g = Graph.er(1 << 14, 1 << 17, seed=3) if name == "er" else Graph(*csr)
+        ctx = Context(g)
+        ctx.load_policy(w)
+        for s in (0, g.n // 2, g.n - 1):
+            for pers in (1, 0):
+                lv, _, rl, _ = ctx.bfs(s, rec_cap=128, policy=POLICY_LEARNED, persistent=pers)
+                _, _, rh, _ = ctx.bfs(s, rec_cap=128, policy=POLICY_HEURISTIC, persistent=pers)
+                want, _, _ = O.bfs_queue(g.beg_pos, g.adj, s)
+                assert (lv == want).all()
+                if name == "star_path":
+                    assert len(rl) > 16
+                    assert all(r["policy"] == POLICY_LEARNED for r in rl if r["level"] <= 15)
+                    assert all(r["policy"] == 0 for r in rl if r["level"] > 15)
+                else:
+                    assert all(r["policy"] == 0 for r in rl)
+                    assert [r["mode"] for r in rl] == [r["mode"] for r in rh]
+        ctx.close()

@@ -186,6 +186,7 @@ int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, i
         }
     }
     c->n_nz = nz;
+    c->skew = nz > 0 && m > 0 ? (double)max_deg * (double)nz / (double)m : 0.0;
     auto fail = [&](int code) {
         xb_ctx_free_device(c);
         delete c;
@@ -328,7 +329,10 @@ static XbArgs make_args(xbfs_ctx *c, int32_t src, const xbfs_opts &o)
     a.rec_cap = XB_REC_CAP;
     a.sched = c->d_sched;
     a.sched_len = o.policy == XBFS_POLICY_SCHEDULE ? o.schedule_len : 0;
-    a.mlp = (o.policy == XBFS_POLICY_LEARNED && c->mlp_loaded) ? c->d_mlp : nullptr;
+    // the learned policy decides only on graphs of the skew it was trained on
+    a.mlp = (o.policy == XBFS_POLICY_LEARNED && c->mlp_loaded && c->skew >= XBFS_MLP_MIN_SKEW)
+                ? c->d_mlp
+                : nullptr;
     a.n = c->n;
     a.m = c->m;
     a.nwords = c->nwords;
@@ -445,7 +449,7 @@ static int bfs_host_loop(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs_
         if (L >= a.max_levels) return XBFS_ERR_LIMIT;
         float y[XB_MLP_NO];
         const float *pred = nullptr;
-        if (a.mlp) {
+        if (a.mlp && L <= XBFS_MLP_MAX_LEVEL) {
             float x[XB_MLP_NI];
             xbfs_mlp_features(L, n_f, m_f, n_u, m_u, s.prev_mode, x);
             xbfs_mlp_forward1(c->h_mlp.data(), x, y);
@@ -1125,7 +1129,7 @@ extern "C" int xbfs_bfs_probe(xbfs_ctx *c, int32_t src, const xbfs_opts *opts, i
         }
         float y[XB_MLP_NO];
         const float *pred = nullptr;
-        if (a.mlp) {
+        if (a.mlp && L <= XBFS_MLP_MAX_LEVEL) {
             float x[XB_MLP_NI];
             xbfs_mlp_features(L, n_f, m_f, n_u, m_u, s.prev_mode, x);
             xbfs_mlp_forward1(c->h_mlp.data(), x, y);

@@ -22,6 +22,7 @@ struct xbfs_ctx {
     int64_t ntiles_bu = 0;      // bottom-up tiles (XB_BU_TILE_WORDS words each)
     int64_t fstride = 0;        // words per frontier ring (tile multiple)
     int64_t n_nz = 0;           // vertices with degree > 0
+    double skew = 0;            // largest degree / mean degree of the non-isolated vertices
     int64_t *d_beg = nullptr;   // int64[n+1]
     int32_t *d_adj = nullptr;   // int32[m]
     uint64_t *d_nz = nullptr;   // bit v: deg(v) > 0

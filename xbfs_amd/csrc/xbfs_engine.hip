@@ -1101,7 +1101,7 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
             xb_wave_sync();
             if (ls.open) {
                 const float *pred = nullptr;
-                if (a.mlp) {
+                if (a.mlp && L <= XBFS_MLP_MAX_LEVEL) {
                     if (threadIdx.x == 0)
                         xbfs_mlp_features(L, ls.n_f, ls.m_f, ls.n_u, ls.m_u, ls.st.prev_mode,
                                           sm.mlp.x);

@@ -117,6 +117,14 @@ XBFS_HD inline int32_t xbfs_mode_to_tag(int32_t mode)
 #define XB_MLP_NI 8
 #define XB_MLP_NH 32
 #define XB_MLP_NO 3
+// Domain of the trained model.  It was trained on skewed RMAT graphs (the
+// training family's largest degree is 363-1366 times the mean degree of the
+// non-isolated vertices) and on levels below 16 (the level feature L/16 stays
+// below 1).  On low-skew graphs (Erdos-Renyi about 2.4, grids about 1) the
+// modes' costs follow other rules, and deeper levels lie outside the features
+// it has seen: there the heuristic decides, and the level records say so.
+#define XBFS_MLP_MIN_SKEW 16.0
+#define XBFS_MLP_MAX_LEVEL 15
 #define XB_MLP_NPARAM (XB_MLP_NH * XB_MLP_NI + XB_MLP_NH + XB_MLP_NH * XB_MLP_NH + XB_MLP_NH + \
                        XB_MLP_NO * XB_MLP_NH + XB_MLP_NO)
 
@@ -198,7 +206,7 @@ XBFS_HD inline int32_t xbfs_choose_mode(int32_t policy, const int8_t *sched, int
     case 4: return 2;
     case 5: return sched[L < sched_len ? L : sched_len - 1];
     case 1:
-        if (pred) {
+        if (pred && L <= XBFS_MLP_MAX_LEVEL) {
             bool finite = true;
             for (int j = 0; j < XB_MLP_NO; j++)
                 if (!(pred[j] > -1e30f && pred[j] < 1e30f)) finite = false;
