@@ -91,11 +91,15 @@ def init_params(seed=0):
     return p
 
 
-def train(X, Y, M, steps=4000, lr=3e-3, seed=0):
-    """Full-batch Adam on the GPU (library kernels).  Returns (params, curve)."""
-    # start the output biases at the label means so early steps fit structure
-    p = init_params(seed)
-    p[-3:] = (Y * M).sum(0) / np.maximum(M.sum(0), 1)
+def train(X, Y, M, steps=4000, lr=3e-3, seed=0, init=None):
+    """Full-batch Adam on the GPU (library kernels).  Returns (params, curve).
+    init: starting weights (fine-tuning); default a fresh initialisation."""
+    if init is not None:
+        p = np.array(init, dtype=np.float32, copy=True)
+    else:
+        # start the output biases at the label means so early steps fit structure
+        p = init_params(seed)
+        p[-3:] = (Y * M).sum(0) / np.maximum(M.sum(0), 1)
     return mlp_train(p, X, Y, M, steps, lr)
 
 
