@@ -16,8 +16,14 @@ def test_features_closed_form():
     x = C.policy_features(0, 0, 0, 0, 0, -1)
     assert np.allclose(x, [0, 0, 0, 0, 0, -1, 0, 1])
     x = C.policy_features(3, 1023, 4095, (1 << 20) - 1, 15, 2)
-    # log2(1+v)/32 at exact powers of two, level/16, previous mode BU
-    assert np.allclose(x, [10 / 32, 12 / 32, 4 / 32, 20 / 32, 3 / 16, 1, 0, 1], atol=2e-5)
+    # log2(1+v)/32 at exact powers of two; frontier-to-unvisited log2 ratios of
+    # edges (12 - 4) and vertices (10 - 20); level/16, previous mode BU
+    assert np.allclose(x, [10 / 32, 12 / 32, 8 / 32, -10 / 32, 3 / 16, 1, 0, 1], atol=2e-5)
+    # the ratios do not depend on the graph's size: scaling every count by 2^10
+    # leaves them unchanged (up to the +1 inside the logarithm)
+    a = C.policy_features(2, 1 << 12, 1 << 20, 1 << 22, 1 << 28, 0)
+    b = C.policy_features(2, 1 << 22, 1 << 30, 1 << 32, 1 << 38, 0)
+    assert np.allclose(a[2:4], b[2:4], atol=1e-4) and abs(b[0] - a[0] - 10 / 32) < 1e-4
     x = C.policy_features(1, 5, 5, 5, 5, 0)
     assert x[5] == 0 and x[6] == 1
     # log2(1+v) approximation stays within 1e-4 of the true value
@@ -191,3 +197,17 @@ def test_learned_policy_domain_guard():
                     assert all(r["policy"] == 0 for r in rl)
                     assert [r["mode"] for r in rl] == [r["mode"] for r in rh]
         ctx.close()
+
+
+def test_committed_samples_match_library_features():
+    """The committed training samples carry their raw level states; their
+    stored features must be what the library computes from those states, so
+    the committed weights and the feature code cannot drift apart."""
+    path = os.path.join(os.path.dirname(P.WEIGHTS), "policy_samples.npz")
+    if not os.path.exists(path):
+        pytest.skip("no committed samples")
+    d = np.load(path)
+    assert set(d.files) >= {"X", "Y", "M", "R"}
+    idx = np.random.default_rng(0).choice(d["R"].shape[0], 500, replace=False)
+    assert np.allclose(P.features_of(d["R"][idx]), d["X"][idx], atol=1e-6)
+    assert d["Y"].shape == (d["X"].shape[0], 3) and np.isfinite(d["Y"]).all()

@@ -146,16 +146,20 @@ XBFS_HD inline float xb_log2p1(long long v)
     return (float)(e + l);
 }
 
-// Features of level L: log2 sizes of the frontier and the unvisited part,
-// the level index, the previous direction and whether an exact queue is
-// available (previous level was top-down scan-free), and a constant.
+// Features of level L: log2 sizes of the frontier (vertices, edges), the
+// log2 ratios of the frontier to the unvisited part (edges, vertices), the
+// level index, the previous direction and whether an exact queue is available
+// (previous level was top-down scan-free), and a constant.  The two ratios
+// carry the direction decision independently of the graph's size (the
+// unvisited counts alone grow with the graph, beyond the training graphs').
 XBFS_HD inline void xbfs_mlp_features(int32_t L, long long n_f, long long m_f, long long n_u,
                                       long long m_u, int32_t prev_mode, float *x)
 {
-    x[0] = xb_log2p1(n_f) * (1.0f / 32.0f);
-    x[1] = xb_log2p1(m_f) * (1.0f / 32.0f);
-    x[2] = xb_log2p1(m_u) * (1.0f / 32.0f);
-    x[3] = xb_log2p1(n_u) * (1.0f / 32.0f);
+    const float lnf = xb_log2p1(n_f), lmf = xb_log2p1(m_f);
+    x[0] = lnf * (1.0f / 32.0f);
+    x[1] = lmf * (1.0f / 32.0f);
+    x[2] = (lmf - xb_log2p1(m_u)) * (1.0f / 32.0f);
+    x[3] = (lnf - xb_log2p1(n_u)) * (1.0f / 32.0f);
     x[4] = (float)(L < 64 ? L : 64) * (1.0f / 16.0f);
     x[5] = prev_mode == 2 ? 1.0f : (prev_mode < 0 ? -1.0f : 0.0f);
     x[6] = prev_mode == 0 ? 1.0f : 0.0f;

@@ -49,17 +49,28 @@ def seeded_sources(deg, k, seed=42):
     return rng.choice(cand, size=min(k, cand.size), replace=False).astype(np.int32)
 
 
+RAW_FIELDS = ("level", "n_f", "m_f", "n_u", "m_u", "prev_mode")
+
+
 def records_to_samples(recs):
-    X, Y = [], []
+    """Probe records -> (features, labels, raw level state).  The raw state
+    (RAW_FIELDS per level) lets recorded samples be re-featurised later."""
+    X, Y, R = [], [], []
     for r in recs:
-        X.append(policy_features(r["level"], r["n_f"], r["m_f"], r["n_u"], r["m_u"],
-                                 r["prev_mode"]))
+        X.append(policy_features(*(r[k] for k in RAW_FIELDS)))
         Y.append([np.log2(max(t, 1e-7) * 1e6) for t in r["t_mode"]])
-    return X, Y
+        R.append([r[k] for k in RAW_FIELDS])
+    return X, Y, R
+
+
+def features_of(R):
+    """Features of recorded raw level states, by the library's definition."""
+    return np.asarray([policy_features(*(int(v) for v in row)) for row in R], np.float32)
 
 
 def collect(specs, sources=4, reps=3, log=None):
-    X, Y = [], []
+    """Returns (X, Y, M, R): features, labels, label mask, raw level states."""
+    X, Y, R = [], [], []
     for k, sp in enumerate(specs):
         t0 = time.time()
         g = Graph.rmat(sp["scale"], sp["ef"], sp["a"], sp["b"], sp["c"], seed=sp["seed"])
@@ -67,9 +78,10 @@ def collect(specs, sources=4, reps=3, log=None):
         srcs = seeded_sources(g.degrees(), sources, seed=1000 + sp["seed"])
         for j, s in enumerate(srcs):
             recs, _ = ctx.probe(int(s), follow_best=(j % 2 == 1), reps=reps)
-            x, y = records_to_samples(recs)
+            x, y, raw = records_to_samples(recs)
             X += x
             Y += y
+            R += raw
         ctx.close()
         del g
         if log:
@@ -77,7 +89,7 @@ def collect(specs, sources=4, reps=3, log=None):
                 f"{len(X)} samples ({time.time() - t0:.1f}s)")
     X = np.asarray(X, np.float32)
     Y = np.asarray(Y, np.float32)
-    return X, Y, np.ones_like(Y)
+    return X, Y, np.ones_like(Y), np.asarray(R, np.int64)
 
 
 def init_params(seed=0):

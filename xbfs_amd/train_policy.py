@@ -33,14 +33,19 @@ def main():
     t0 = time.time()
     if args.data:
         d = np.load(args.data)
-        X, Y, M = d["X"], d["Y"], d["M"]
+        Y, M = d["Y"], d["M"]
+        R = d["R"] if "R" in d.files else None
+        # recorded raw level states are featurised by the library's current
+        # definition; samples without them keep their stored features
+        X = policy.features_of(R) if R is not None else d["X"]
     else:
         specs = policy.train_specs(args.graphs)
-        X, Y, M = policy.collect(specs, sources=args.sources, reps=args.reps,
-                                 log=lambda s: print(s, flush=True))
+        X, Y, M, R = policy.collect(specs, sources=args.sources, reps=args.reps,
+                                    log=lambda s: print(s, flush=True))
     t_collect = time.time() - t0
     if args.out_data:
-        np.savez_compressed(args.out_data, X=X, Y=Y, M=M)
+        extra = {"R": R} if R is not None else {}
+        np.savez_compressed(args.out_data, X=X, Y=Y, M=M, **extra)
     init = policy.load(args.init) if args.init else None
     params, curve = policy.train(X, Y, M, steps=args.steps, lr=args.lr, init=init)
     pred_mode = None
