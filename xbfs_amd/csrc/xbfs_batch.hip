@@ -161,8 +161,15 @@ __global__ void __launch_bounds__(XB_B64_BLOCK) k_b64_push(XbB64Args a)
 }
 
 // Bottom-up: lane per segment of a vertex with unvisited bits; neighbours in
-// groups of 4 (independent gathers), early exit once the unvisited bits are
-// all covered.
+// rounds of XB_B64_PULL_K (all adjacency words of a round loaded, then all
+// their frontier masks gathered, each group independent, so a round costs two
+// dependent memory trips), early exit once the unvisited bits are all
+// covered.  On the level where most vertices still miss some source, nearly
+// every segment is scanned whole, and the number of trips per segment bounds
+// the level.
+#ifndef XB_B64_PULL_K
+#define XB_B64_PULL_K 16
+#endif
 __global__ void __launch_bounds__(XB_B64_BLOCK) k_b64_pull(XbB64Args a)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -171,15 +178,18 @@ __global__ void __launch_bounds__(XB_B64_BLOCK) k_b64_pull(XbB64Args a)
         const uint64_t U = a.all & ~a.vis_cur[v];
         if (!U) continue;
         const int64_t e1 = a.seg_e[i + 1];
-        int64_t e = a.seg_e[i];
         uint64_t got = 0;
-        for (; e + 4 <= e1; e += 4) {
-            const int32_t u0 = a.adj[e], u1 = a.adj[e + 1], u2 = a.adj[e + 2], u3 = a.adj[e + 3];
-            got |= a.F_cur[u0] | a.F_cur[u1] | a.F_cur[u2] | a.F_cur[u3];
+        for (int64_t e = a.seg_e[i]; e < e1; e += XB_B64_PULL_K) {
+            int32_t u[XB_B64_PULL_K];
+#pragma unroll
+            for (int k = 0; k < XB_B64_PULL_K; k++) u[k] = e + k < e1 ? a.adj[e + k] : -1;
+            uint64_t f[XB_B64_PULL_K];
+#pragma unroll
+            for (int k = 0; k < XB_B64_PULL_K; k++) f[k] = u[k] >= 0 ? a.F_cur[u[k]] : 0ull;
+#pragma unroll
+            for (int k = 0; k < XB_B64_PULL_K; k++) got |= f[k];
             if ((got & U) == U) break;
         }
-        if ((got & U) != U)
-            for (; e < e1; e++) got |= a.F_cur[a.adj[e]];
         const uint64_t m = got & U;
         if (m) atomicOr(reinterpret_cast<unsigned long long *>(&a.acc[v]), (unsigned long long)m);
     }
