@@ -14,6 +14,9 @@ reference heuristic and, when trained weights are present
 (xbfs_amd/data/policy_mlp.f32), under the learned policy; value is the
 learned policy's figure and "policies" reports both, their ratio, and the
 per-level oracle-best schedule measured on a subset of the sources.
+"without_component_end" times the same sources on a context built with
+XBFS_COMPONENT=off (every traversal expands levels until its frontier is
+empty instead of stopping once the source's component is covered).
 
 roofline: algorithmic bytes of one traversal, B = 8*m_cc + 20*V_cc + 4*V
 (int32 adjacency word + int32 level probe per traversed edge; int64 offset
@@ -144,6 +147,8 @@ def main():
                     help="sources timed for the single-core serial reference path (0 = skip)")
     ap.add_argument("--oracle-best", type=int, default=16,
                     help="sources on which the per-level oracle-best schedule is measured")
+    ap.add_argument("--no-component", type=int, default=1,
+                    help="also time the sources with XBFS_COMPONENT=off (reported beside)")
     ap.add_argument("--compare", type=int, default=1,
                     help="1 = also time the heuristic policy; 0 = only the main policy "
                          "(profiling runs, so every k_persistent launch is the measured one)")
@@ -243,6 +248,33 @@ def main():
                 "learned_ms": round(tot["learned"] * 1e3, 3),
                 "oracle_best_ms": round(tot["oracle_best"] * 1e3, 3)}
 
+    # The same sources on a context built with XBFS_COMPONENT=off, where every
+    # traversal runs until its frontier is empty (no create-time component of
+    # the largest-degree vertex): reported beside the headline.
+    no_comp = None
+    if args.no_component:
+        prev = os.environ.get("XBFS_COMPONENT")
+        os.environ["XBFS_COMPONENT"] = "off"
+        try:
+            ctx_nc = Context(g, device=0)
+        finally:
+            if prev is None:
+                del os.environ["XBFS_COMPONENT"]
+            else:
+                os.environ["XBFS_COMPONENT"] = prev
+        if have_model:
+            ctx_nc.load_policy(P.load())
+        t_nc = np.zeros(len(srcs))
+        for _ in range(max(args.steps, 1)):
+            for i, s in enumerate(srcs):
+                t_nc[i] += ctx_nc.bfs(int(s), opts=opts, want_levels=False)[3]["t_bfs_s"]
+        t_nc /= max(args.steps, 1)
+        ctx_nc.close()
+        no_comp = {"gteps_hmean": round(hmean(m_cc / t_nc / 1e9), 3),
+                   "total_ms": round(float(t_nc.sum()) * 1e3, 3),
+                   "note": "XBFS_COMPONENT=off: no create-time component, every traversal "
+                           "expands levels until its frontier is empty"}
+
     # correctness spot check of one source against the oracle validator
     from oracle import oracle as O
     s0 = int(srcs[0])
@@ -296,6 +328,7 @@ def main():
                               else "heuristic (reference alpha rule, alpha=0.1)"),
                    "levels_valid": bool(valid)},
         "policies": policy_cmp,
+        "without_component_end": no_comp,
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
                      "traffic": None if traffic is None else round(traffic),
