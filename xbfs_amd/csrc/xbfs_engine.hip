@@ -823,11 +823,14 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
         // consumed count, so no unread entry is overwritten) and finished by
         // the whole wave below.  When every row starts with its hub (rows
         // sorted by neighbour degree), the scan starts after it: the hub pass
-        // probed it.  Its level-(L+1) bit then goes unchecked, which only
-        // leaves a +2 discovery to the next level.
+        // probed it.  Scans probe F_L only: a candidate with no level-L
+        // neighbour stays unvisited and is found at the next level.  (Probing
+        // F_{L+1} as well, to settle such candidates at L+2 now, costs a
+        // dependent round trip per step of every full scan, more than it
+        // saves at the next level.)
         const int64_t skip = a.hub_first ? 1 : 0;
         int ndef = 0, next = 0;
-        int32_t v = -1, seen = -1;
+        int32_t v = -1;
         int64_t j = 0, ce = 0, e = 0, b = 0;
         for (;;) {
             const bool idle = v < 0;
@@ -840,7 +843,6 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                     e = a.beg[v + 1];
                     j = b + skip;
                     ce = (e - j > XB_BU_LANE_MAX) ? j + XB_BU_LANE_MAX : e;
-                    seen = -1;
                 }
             }
             next = min(nres, next + (int)__popcll(im));
@@ -867,24 +869,9 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                     m1 += (unsigned long long)(e - b);
                     v = -1;
                 } else {
-                    if (seen < 0) {
-                        if ((F1_32[u0 >> 5] >> (u0 & 31)) & 1u) seen = u0;
-                        else if (u1 >= 0 && ((F1_32[u1 >> 5] >> (u1 & 31)) & 1u)) seen = u1;
-                        else if (u2 >= 0 && ((F1_32[u2 >> 5] >> (u2 & 31)) & 1u)) seen = u2;
-                        else if (u3 >= 0 && ((F1_32[u3 >> 5] >> (u3 & 31)) & 1u)) seen = u3;
-                    }
                     j += 4;
                     if (j >= ce) {
-                        if (ce < e) {
-                            defer = true; // unscanned edges remain: no +2 decision yet
-                        } else if (seen >= 0) {
-                            __builtin_nontemporal_store(L2, &a.level[v]);
-                            if (a.parent) a.parent[v] = seen;
-                            atomicOr(&wl->f2[li], bit);
-                            atomicOr(&wl->va[li], bit);
-                            n2 += 1;
-                            m2 += (unsigned long long)(e - b);
-                        }
+                        defer = ce < e; // unscanned edges remain: the wave finishes them
                         v = -1;
                     }
                 }
