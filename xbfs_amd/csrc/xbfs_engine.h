@@ -128,7 +128,6 @@ union XbSmem {
 struct XbWaveLds {
     int32_t list[XB_CHUNK_WORDS * 64];      // the chunk's vertices to process
     unsigned long long f1[XB_CHUNK_WORDS];  // bottom-up: new level-(L+1) bits
-    unsigned long long f2[XB_CHUNK_WORDS];  // bottom-up: new level-(L+2) bits (+2 rule)
     unsigned long long va[XB_CHUNK_WORDS];  // bottom-up: new visited bits
 };
 

@@ -3,10 +3,11 @@
 // Frontier representation.  Every operation that assigns level l to a vertex
 // also sets its bit in the frontier bitmap of ring slot l & 3 and in the
 // visited bitmap.  At the start of level L the bitmap F_L therefore holds
-// exactly the level-L vertices, whichever mode discovered them (including
-// vertices the bottom-up +2 rule set two levels early).  A binned queue of F_L
-// is available for free when level L-1 ran top-down scan-free and no +2
-// discovery targets level L; otherwise a ballot sweep of F_L builds it.
+// exactly the level-L vertices, whichever mode discovered them.  A binned
+// queue of F_L is available for free when level L-1 ran top-down scan-free;
+// otherwise a ballot sweep of F_L builds it.  (The statistics keep fields for
+// discoveries two levels ahead, the reference's bottom-up +2 rule; no phase
+// makes them, so they stay zero.)
 //
 // Phases (device functions run by every block of a grid, with a grid-wide
 // barrier between phases):
@@ -24,8 +25,9 @@
 //   ph_bu        bottom-up over unvisited vertices (nz & ~visited), compacted
 //                per wave-owned 1024-vertex chunk in LDS by popcount and a wave scan
 //                (ballot_counting/FQ_gen/prefix sum, bfs_adaptiveFQ_wb_async.cuh:
-//                189-284, prefix_sum1.cuh) and updated with the asynchronous
-//                +2 rule (thread_centric_BU_update, bfs_adaptiveFQ_wb_async.cuh:738-777)
+//                189-284, prefix_sum1.cuh), each candidate settled by its first
+//                level-L neighbour (thread_centric_BU_update,
+//                bfs_adaptiveFQ_wb_async.cuh:738-777, without its +2 rule)
 // Drivers: k_persistent runs the whole level loop in one cooperative launch
 // with a grid barrier; the host loop launches one kernel per phase.
 #include <hip/hip_runtime.h>
@@ -614,10 +616,9 @@ __device__ void td_concat(const XbArgs &a, const XbTdCtx &t, int32_t cnt, Sel se
 }
 
 // carry (persistent kernel, LDS; NULL elsewhere): what this block stored in
-// the (count, degree sum) fields of ring slot L+1 at level L-1 (its +2
-// discoveries), so the counts of level L are stored without reading the
-// column back; set to zero here, since a top-down level makes no +2
-// discoveries.
+// the (count, degree sum) fields of ring slot L+1 at level L-1 (discoveries
+// two levels ahead; none is made, so it is zero), so the counts of level L are
+// stored without reading the column back; set to zero here.
 template <bool SF>
 __device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue, XbSmem &sm,
                       long long *carry = nullptr)
@@ -698,8 +699,8 @@ __device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue,
 // ---------------------------------------------------------------------------
 // bottom-up
 // ---------------------------------------------------------------------------
-// carry: as for ph_td; afterwards it holds this block's +2 totals, stored in
-// ring slot L+2.
+// carry: as for ph_td (this phase makes no +2 discoveries either: a
+// candidate without a level-L neighbour stays unvisited for the next level).
 // Level (and top-down single-scan level) stores are nontemporal: each entry is
 // written once per traversal and not read back by its writer, so it need not
 // sit dirty in L2 until the level's release writes it back.
@@ -708,17 +709,15 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
 {
     const uint64_t *F = xb_f(a, L);
     uint64_t *F1 = xb_f(a, L + 1);
-    uint64_t *F2 = xb_f(a, L + 2);
     const uint32_t *F32 = reinterpret_cast<const uint32_t *>(F);
-    const uint32_t *F1_32 = reinterpret_cast<const uint32_t *>(F1);
-    const int32_t L1 = L + 1, L2 = L + 2;
+    const int32_t L1 = L + 1;
     const int ln = xb_lane();
     XbWaveLds *wl = lds + (threadIdx.x >> 6);
     const int64_t gw = ((int64_t)bid * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwv = ((int64_t)nb * blockDim.x) >> 6;
     const int64_t nchunks = (a.nwords + XB_CHUNK_WORDS - 1) / XB_CHUNK_WORDS;
     const uint64_t below = ln ? (~0ull >> (64 - ln)) : 0ull;
-    unsigned long long n1 = 0, m1 = 0, n2 = 0, m2 = 0;
+    unsigned long long n1 = 0, m1 = 0;
     // The wave's chunks gw, gw + W, gw + 2W, ... in groups of 4: lane 16q + k
     // loads word k of the group's chunk q, so one memory round trip covers 4
     // chunks and empty chunks are skipped by one ballot.
@@ -752,18 +751,16 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
         }
         if (ln < XB_CHUNK_WORDS) {
             wl->f1[ln] = 0ull;
-            wl->f2[ln] = 0ull;
             wl->va[ln] = 0ull;
         }
         xb_wave_sync();
         // Hub probe, 4 candidates per lane per round (independent loads): a
         // candidate whose largest-degree neighbour is in F_L gets level L+1
         // with that neighbour as parent, without reading its adjacency.  For
-        // a degree-1 candidate the hub is its only neighbour, so the probe
-        // settles it completely: on a miss, the hub's F_{L+1} bit (loaded
-        // with the F_L word) applies the +2 rule, and either way it needs no
-        // scan.  The others are compacted to the front of the list (positions
-        // below the entries read so far) for the scan below.
+        // a degree-1 candidate the hub is its only neighbour, so a miss
+        // settles its question for this level and it needs no scan.  The
+        // others are compacted to the front of the list (positions below the
+        // entries read so far) for the scan below.
         int nres = 0;
         for (int r = 0; r < total; r += 256) {
             int32_t vc[4];
@@ -777,32 +774,23 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
 #pragma unroll
             for (int t = 0; t < 4; t++) h[t] = vc[t] >= 0 ? a.hd[vc[t]] : make_int2(-1, 0);
             // 32-bit probes: the dword of the bitmap word that holds the bit
-            uint32_t fw[4], f1w[4];
+            uint32_t fw[4];
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                fw[t] = h[t].x >= 0 ? F32[h[t].x >> 5] : 0u;
-                f1w[t] = (h[t].x >= 0 && h[t].y == 1) ? F1_32[h[t].x >> 5] : 0u;
-            }
+            for (int t = 0; t < 4; t++) fw[t] = h[t].x >= 0 ? F32[h[t].x >> 5] : 0u;
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 hit[t] = (fw[t] >> (h[t].x & 31)) & 1u;
                 done[t] = hit[t] || h[t].y == 1;
-                const bool p2 = !hit[t] && h[t].y == 1 && ((f1w[t] >> (h[t].x & 31)) & 1u);
-                if (hit[t] || p2) {
+                if (hit[t]) {
                     const int32_t v = vc[t];
                     const unsigned long long bit = 1ull << (v & 63);
                     const int li = (int)((v >> 6) - w0);
-                    __builtin_nontemporal_store(hit[t] ? L1 : L2, &a.level[v]);
+                    __builtin_nontemporal_store(L1, &a.level[v]);
                     if (a.parent) a.parent[v] = h[t].x;
-                    atomicOr(hit[t] ? &wl->f1[li] : &wl->f2[li], bit);
+                    atomicOr(&wl->f1[li], bit);
                     atomicOr(&wl->va[li], bit);
-                    if (hit[t]) {
-                        n1 += 1;
-                        m1 += (unsigned long long)h[t].y;
-                    } else {
-                        n2 += 1;
-                        m2 += 1;
-                    }
+                    n1 += 1;
+                    m1 += (unsigned long long)h[t].y;
                 }
             }
             xb_wave_sync();
@@ -886,8 +874,8 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
         }
         // The deferred candidates, one at a time by the whole wave: 256 edges
         // per round (4 coalesced adjacency words per lane), a ballot ends the
-        // scan at the first level-L neighbour.  The +2 rule is not applied to
-        // them: one left unvisited is found at the next level.
+        // scan at the first level-L neighbour; one without such a neighbour
+        // stays unvisited and is found at the next level.
         for (int d = 0; d < ndef; d++) {
             const int32_t dv = wl->list[d];
             const int64_t db = a.beg[dv], de = a.beg[dv + 1];
@@ -931,24 +919,18 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
             if (va) {
                 a.vis[w] |= va;
                 const unsigned long long f1 = wl->f1[ln];
-                if (f1) F1[w] |= f1; // other chunks' +2 probes see it from here on
-                const unsigned long long f2 = wl->f2[ln];
-                if (f2) F2[w] |= f2;
+                if (f1) F1[w] |= f1;
             }
         }
     }
     }
-    // n2 also goes to the +2 field: a top-down level L+1 must not trust its winner
-    // queue as the level-(L+2) frontier when +2 discoveries exist
-    unsigned long long vv[5] = {n1, m1, n2, m2, n2};
-    const int ff[5] = {xb_f_ring(L1, 0), xb_f_ring(L1, 1), xb_f_ring(L2, 0), xb_f_ring(L2, 1),
-                       xb_f_ring(L2, 2)};
-    // ring slot L+2 was zeroed at level L-1 and nothing wrote it since
-    long long old[5] = {carry ? carry[0] : 0, carry ? carry[1] : 0, 0, 0, 0}, tot[5];
-    xb_part_add<5>(a, bid, vv, ff, carry != nullptr, old, tot);
+    unsigned long long vv[2] = {n1, m1};
+    const int ff[2] = {xb_f_ring(L1, 0), xb_f_ring(L1, 1)};
+    long long old[2] = {carry ? carry[0] : 0, carry ? carry[1] : 0}, tot[2];
+    xb_part_add<2>(a, bid, vv, ff, carry != nullptr, old, tot);
     if (carry && threadIdx.x == 0) {
-        carry[0] = tot[2];
-        carry[1] = tot[3];
+        carry[0] = 0;
+        carry[1] = 0;
     }
 }
 
