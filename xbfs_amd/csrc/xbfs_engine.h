@@ -10,7 +10,10 @@
 #define XB_QBUFS 3              /* queue buffers, ring by level % 3               */
 #define XB_FRINGS 4             /* frontier bitmaps, ring by level & 3            */
 #define XB_BU_TILE_WORDS 32     /* bitmap words per bottom-up tile (2048 vertices) */
-#define XB_GCAP 512             /* grid-bin vertices scanned per block pass       */
+#ifndef XB_GCAP
+#define XB_GCAP 2048            /* vertices per concatenated top-down pass, at most; a pass
+                                   takes 2 per thread, so 2 * blockDim.x bounds it too */
+#endif
 #define XB_PBLOCK 256           /* block size of the per-phase kernels            */
 #define XB_PERSIST_BLOCK 1024   /* block size of the persistent kernel (1 per CU) */
 #ifndef XB_BU_LANE_MAX

@@ -567,7 +567,7 @@ __device__ __forceinline__ void td_visit(const XbArgs &a, const XbTdCtx &t, int3
     }
 }
 
-// Visits the rows of cnt <= XB_GCAP frontier vertices sel(0..cnt-1) as one
+// Visits the rows of cnt <= min(XB_GCAP, 2 * blockDim.x) frontier vertices sel(0..cnt-1) as one
 // concatenated adjacency shared by every thread of the active grid: each block
 // scans their degrees in LDS (with their row starts), and a thread maps an
 // edge position to its vertex by binary search.  Every thread of the block
@@ -579,7 +579,7 @@ __device__ void td_concat(const XbArgs &a, const XbTdCtx &t, int32_t cnt, Sel se
 {
     long long *s_pref = sm.td.pref, *s_b = sm.td.b;
     int32_t *s_u = sm.td.u;
-    // two elements per thread (blockDim.x >= 256, so 2 * blockDim.x >= XB_GCAP)
+    // two elements per thread (cnt <= 2 * blockDim.x)
     const int k0 = 2 * threadIdx.x, k1 = k0 + 1;
     long long d0 = 0, d1 = 0;
     if (k0 < cnt) {
@@ -639,17 +639,20 @@ __device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue,
     const int64_t nthr = (int64_t)nb * blockDim.x;
     const int ln = xb_lane();
 
-    // A frontier of at most XB_GCAP vertices, whatever their bins, is one
+    // A frontier of at most `cap` vertices, whatever their bins, is one
     // concatenated adjacency for the whole grid (no lane or wave walks a row
     // serially, so a level with a few hubs or a few dozen mid-degree vertices
     // takes about one round of dependent loads); otherwise the grid bin is,
-    // XB_GCAP vertices per pass, and the other bins follow.
+    // `cap` vertices per pass, and the other bins follow.  cap is XB_GCAP or
+    // two per thread of the block, whichever is smaller (512 for the per-phase
+    // kernels' 256-thread blocks, 2048 for the persistent kernel's 1024).
     const int32_t nall = t.nc[0] + t.nc[1] + t.nc[2] + t.nc[3];
-    const bool flat = nall <= XB_GCAP;
+    const int32_t cap = min(XB_GCAP, 2 * (int)blockDim.x);
+    const bool flat = nall <= cap;
     const int32_t c0 = t.nc[0], c1 = c0 + t.nc[1], c2 = c1 + t.nc[2];
     const int32_t ncat = flat ? nall : t.nc[3];
-    for (int32_t base = 0; base < ncat; base += XB_GCAP) {
-        td_concat<SF>(a, t, min(XB_GCAP, ncat - base), [&](int32_t k) {
+    for (int32_t base = 0; base < ncat; base += cap) {
+        td_concat<SF>(a, t, min(cap, ncat - base), [&](int32_t k) {
             if (!flat) return t.qc[3][base + k];
             return k < c0 ? t.qc[0][k] : k < c1 ? t.qc[1][k - c0] : k < c2 ? t.qc[2][k - c1]
                                                                             : t.qc[3][k - c2];
