@@ -17,7 +17,7 @@
 #define XB_PBLOCK 256           /* block size of the per-phase kernels            */
 #define XB_PERSIST_BLOCK 1024   /* block size of the persistent kernel (1 per CU) */
 #ifndef XB_BU_LANE_MAX
-#define XB_BU_LANE_MAX 64       /* bottom-up: edges one lane scans before the wave takes over */
+#define XB_BU_LANE_MAX 32       /* bottom-up: edges one lane scans before the wave takes over */
 #endif
 
 // Control block in device memory.  Ring slots are indexed by level & 3,
