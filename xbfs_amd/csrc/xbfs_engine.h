@@ -14,6 +14,7 @@
 #define XB_GCAP 2048            /* vertices per concatenated top-down pass, at most; a pass
                                    takes 2 per thread, so 2 * blockDim.x bounds it too */
 #endif
+#define XB_GCAP_SMALL 512       /* the same on small-frontier levels (16 active blocks) */
 #define XB_PBLOCK 256           /* block size of the per-phase kernels            */
 #define XB_PERSIST_BLOCK 1024   /* block size of the persistent kernel (1 per CU) */
 #ifndef XB_BU_LANE_MAX

@@ -643,11 +643,14 @@ __device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue,
     // concatenated adjacency for the whole grid (no lane or wave walks a row
     // serially, so a level with a few hubs or a few dozen mid-degree vertices
     // takes about one round of dependent loads); otherwise the grid bin is,
-    // `cap` vertices per pass, and the other bins follow.  cap is XB_GCAP or
-    // two per thread of the block, whichever is smaller (512 for the per-phase
-    // kernels' 256-thread blocks, 2048 for the persistent kernel's 1024).
+    // `cap` vertices per pass, and the other bins follow.  cap is at most two
+    // per thread of the block (512 in the per-phase kernels' 256-thread
+    // blocks): XB_GCAP when the whole grid runs the level, XB_GCAP_SMALL on
+    // small-frontier levels, where only XB_SMALL_BLOCKS blocks share the
+    // edges and each block's scan of the pass's degrees weighs more (a 2048
+    // cap there made the 4096^2 grid 5 % slower).
     const int32_t nall = t.nc[0] + t.nc[1] + t.nc[2] + t.nc[3];
-    const int32_t cap = min(XB_GCAP, 2 * (int)blockDim.x);
+    const int32_t cap = min(nb > XB_SMALL_BLOCKS ? XB_GCAP : XB_GCAP_SMALL, 2 * (int)blockDim.x);
     const bool flat = nall <= cap;
     const int32_t c0 = t.nc[0], c1 = c0 + t.nc[1], c2 = c1 + t.nc[2];
     const int32_t ncat = flat ? nall : t.nc[3];
