@@ -14,7 +14,7 @@
 #define XB_GCAP 2048            /* vertices per concatenated top-down pass, at most; a pass
                                    takes 2 per thread, so 2 * blockDim.x bounds it too */
 #endif
-#define XB_GCAP_SMALL 512       /* the same on small-frontier levels (16 active blocks) */
+#define XB_GCAP_SMALL 256       /* the same on small-frontier levels (32 active blocks) */
 #define XB_PBLOCK 256           /* block size of the per-phase kernels            */
 #define XB_PERSIST_BLOCK 1024   /* block size of the persistent kernel (1 per CU) */
 #ifndef XB_BU_LANE_MAX
@@ -157,9 +157,13 @@ struct XbLoopShared {
 // whose frontier is at most XB_SMALL_NF vertices and XB_SMALL_MF edges runs
 // on blocks [0, XB_SMALL_BLOCKS) only, with a barrier among those blocks;
 // the other blocks park until a level needs the whole grid.
-#define XB_SMALL_BLOCKS 16
+#define XB_SMALL_BLOCKS 32
+#ifndef XB_SMALL_NF
 #define XB_SMALL_NF 16384
+#endif
+#ifndef XB_SMALL_MF
 #define XB_SMALL_MF 65536
+#endif
 
 __host__ __device__ inline void xb_loop_init(XbLoop *s)
 {

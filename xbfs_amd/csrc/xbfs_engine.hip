@@ -647,8 +647,8 @@ __device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue,
     // per thread of the block (512 in the per-phase kernels' 256-thread
     // blocks): XB_GCAP when the whole grid runs the level, XB_GCAP_SMALL on
     // small-frontier levels, where only XB_SMALL_BLOCKS blocks share the
-    // edges and each block's scan of the pass's degrees weighs more (a 2048
-    // cap there made the 4096^2 grid 5 % slower).
+    // edges and each block's scan of the pass's degrees weighs more (on the
+    // 4096^2 grid a 256 cap there is 4 % faster than 2048, 1 % than 512).
     const int32_t nall = t.nc[0] + t.nc[1] + t.nc[2] + t.nc[3];
     const int32_t cap = min(nb > XB_SMALL_BLOCKS ? XB_GCAP : XB_GCAP_SMALL, 2 * (int)blockDim.x);
     const bool flat = nall <= cap;
