@@ -265,6 +265,20 @@ def test_launch_shapes_and_bins(kw):
             run_check(ctx, g, s, **kw, **pol)
 
 
+@pytest.mark.parametrize("grid_dim", [32, 33, 48])
+def test_small_frontier_block_counts(grid_dim):
+    """Persistent grids around the small-frontier block count (32 active
+    blocks): at 32 small levels stay on the whole grid, at 33 one block parks
+    and rejoins on every level that needs it, at 48 sixteen do."""
+    for g in (Graph.grid(64, 64), Graph.rmat(12, 16, seed=1012)):
+        ctx = Context(g)
+        for s in seeded_sources(g, 3):
+            for pol in POLICIES[:4]:
+                say("small blocks", grid_dim, g.n, s, pol)
+                run_check(ctx, g, s, grid_dim=grid_dim, **pol)
+        ctx.close()
+
+
 def test_config1_rmat16_16_sources():
     """RMAT scale 16, ef 16, A/B/C/D = .57/.19/.19/.05, 16 seeded sources."""
     g = Graph.rmat(16, 16, seed=1016)
