@@ -36,6 +36,12 @@ against the sum of the 64 single-source traversals of the same sources
 U{4..32}, seeds 10000+i), one seeded source each; one multi-source traversal
 of the packed graphs against 256 single-source traversals on per-graph
 contexts, with the autotuner's setting for the packed graph.
+
+Other graphs, parity and figures only, never the headline: --graph er
+(Erdos-Renyi, 2^scale vertices, 2^scale * ef / 2 edges) and --graph grid
+(2^(scale/2) x 2^(scale/2) lattice, thousands of levels), e.g.
+`python bench.py --graph grid --sources 4 --cpu-serial 0 --oracle-best 0
+--no-component 0 --batch 0`.
 """
 import argparse
 import json
@@ -135,6 +141,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--graph", choices=("rmat", "er", "grid"), default="rmat",
+                    help="rmat: RMAT(scale, ef), A/B/C/D = .57/.19/.19/.05, seed 1000 + scale; "
+                         "er: Erdos-Renyi, 2^scale vertices and 2^scale * ef / 2 edges, seed 7; "
+                         "grid: 2^(scale/2) x 2^(scale/2) lattice (use --cpu-serial 0 "
+                         "--oracle-best 0: both walk every level)")
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--ef", type=int, default=16)
     ap.add_argument("--sources", type=int, default=64)
@@ -176,7 +187,18 @@ def main():
     from xbfs_amd import Context, Graph, default_opts, POLICY_HEURISTIC, POLICY_LEARNED
     from xbfs_amd import policy as P
     t0 = time.time()
-    g = Graph.rmat(args.scale, args.ef, 0.57, 0.19, 0.19, seed=1000 + args.scale)
+    if args.graph == "rmat":
+        g = Graph.rmat(args.scale, args.ef, 0.57, 0.19, 0.19, seed=1000 + args.scale)
+        gname = f"RMAT-{args.scale} ef{args.ef}"
+        gdesc = f"RMAT scale {args.scale} ef{args.ef} A/B/C/D=.57/.19/.19/.05"
+    elif args.graph == "er":
+        g = Graph.er(1 << args.scale, (1 << args.scale) * args.ef // 2, seed=7)
+        gname = f"ER-{args.scale} ef{args.ef}"
+        gdesc = f"Erdos-Renyi n=2^{args.scale} m=2^{args.scale}*{args.ef}/2 seed 7"
+    else:
+        side = 1 << (args.scale // 2)
+        g = Graph.grid(side, side)
+        gname = gdesc = f"{side}x{side} grid"
     t_gen = time.time() - t0
     deg = g.degrees()
     srcs = pick_sources(deg, args.sources)
@@ -193,7 +215,7 @@ def main():
         if not args.compare:
             pols = pols[1:]
     opts = pols[-1][1]
-    print(f"[bench] RMAT-{args.scale} ef{args.ef}: n={g.n} m={g.m} gen {t_gen:.1f}s; "
+    print(f"[bench] {gname}: n={g.n} m={g.m} gen {t_gen:.1f}s; "
           f"{len(srcs)} sources; policies {[p for p, _ in pols]}", file=sys.stderr, flush=True)
 
     def one_step():
@@ -228,7 +250,7 @@ def main():
     b_alg = 8 * m_cc + 20 * v_cc + 4 * g.n
     achieved = float(b_alg.sum() / t_dev[main_pol].sum())
     traffic = None
-    if PMC_TRAFFIC["workload"] == (args.scale, args.ef, len(srcs), main_pol):
+    if args.graph == "rmat" and PMC_TRAFFIC["workload"] == (args.scale, args.ef, len(srcs), main_pol):
         traffic = 1024.0 * (PMC_TRAFFIC["fetch_kib"] * PMC_TRAFFIC["fetch_factor"] +
                             PMC_TRAFFIC["write_kib"] * PMC_TRAFFIC["write_factor"])
     policy_cmp = {}
@@ -289,7 +311,7 @@ def main():
             lvc, tc = O.omp_td_bfs(g.beg_pos, g.adj, int(s), nthr)
             cg.append(float(deg[lvc >= 0].sum()) / tc / 1e9)
         cpu = {"value": hmean(cg), "unit": "GTEPS", "cores": nthr, "kind": "port",
-               "sample": f"OpenMP top-down BFS, first {len(cg)} of the 64 sources, same graph"}
+               "sample": f"OpenMP top-down BFS, first {len(cg)} of the {len(srcs)} sources, same graph"}
     cpu_serial = None
     if args.cpu_serial > 0:
         cs = []
@@ -300,7 +322,7 @@ def main():
             cs.append(float(deg[lvs >= 0].sum()) / ts / 1e9)
         cpu_serial = {"value": hmean(cs), "unit": "GTEPS", "cores": 1, "kind": "port",
                       "sample": f"reference serial scan BFS (bfs_gpu_update.cu:104-127), "
-                                f"first {len(cs)} of the 64 sources, same graph"}
+                                f"first {len(cs)} of the {len(srcs)} sources, same graph"}
 
     b64 = mg = None
     if args.batch:
@@ -320,9 +342,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
-        "data": "synthetic (seeded RMAT generated in-process)",
-        "config": {"workload": f"RMAT scale {args.scale} ef{args.ef} A/B/C/D=.57/.19/.19/.05, "
-                               f"{len(srcs)} seeded sources, single-source BFS each",
+        "data": f"synthetic ({'seeded ' if args.graph != 'grid' else ''}{args.graph.upper()} "
+                f"graph generated in-process)",
+        "config": {"workload": f"{gdesc}, {len(srcs)} seeded sources, single-source BFS each",
                    "n": g.n, "m": g.m, "persistent": int(opts.persistent),
                    "policy": ("learned MLP cost model (heuristic fallback)" if have_model
                               else "heuristic (reference alpha rule, alpha=0.1)"),
