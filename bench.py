@@ -324,6 +324,8 @@ def main():
                       "sample": f"reference serial scan BFS (bfs_gpu_update.cu:104-127), "
                                 f"first {len(cs)} of the {len(srcs)} sources, same graph"}
 
+    ctx.close()  # before the batch contexts are made, and never left to interpreter teardown
+
     b64 = mg = None
     if args.batch:
         b64, mg = batch_fields()

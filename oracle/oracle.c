@@ -31,6 +31,9 @@
  *                        int32 levels: the CPU baseline.  Pinned by
  *                        agreement with oracle_bfs_queue.
  *   oracle_exclusive_scan  std::exclusive_scan restated (int32 in, int64 out).
+ *   oracle_bfs_queue_rows  oracle_bfs_queue from each of k sources, rows in
+ *                        parallel threads.  Pinned by row-by-row equality
+ *                        with oracle_bfs_queue.
  *   oracle_bfs_multi     serial FIFO BFS from a set of seeds (the batched
  *                        modes' oracle; no reference counterpart).  Pinned by
  *                        the identity level = min over seeds of the
@@ -102,6 +105,18 @@ int32_t oracle_bfs_queue(int64_t n, const int64_t *beg, const int32_t *adj,
     }
     free(q);
     return maxl + 1;
+}
+
+/* k independent single-source BFSs: row i of levels (n entries) is
+ * oracle_bfs_queue from srcs[i].  Rows run in parallel (one OpenMP thread per
+ * row at a time), each exactly the serial FIFO BFS, so full-size batches can
+ * be checked bit for bit in reasonable time. */
+void oracle_bfs_queue_rows(int64_t n, const int64_t *beg, const int32_t *adj,
+                           const int32_t *srcs, int32_t k, int32_t *levels)
+{
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int32_t i = 0; i < k; i++)
+        oracle_bfs_queue(n, beg, adj, srcs[i], levels + (int64_t)i * n, NULL);
 }
 
 /* Serial FIFO BFS from every distinct seed at once (level 0, parent = the

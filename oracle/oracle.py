@@ -43,6 +43,9 @@ def lib():
         L.oracle_omp_td_bfs.argtypes = [c_int64, c_void_p, c_void_p, c_int32, c_void_p, c_int32]
         L.oracle_exclusive_scan.restype = None
         L.oracle_exclusive_scan.argtypes = [c_int64, c_void_p, c_void_p]
+        L.oracle_bfs_queue_rows.restype = None
+        L.oracle_bfs_queue_rows.argtypes = [c_int64, c_void_p, c_void_p, c_void_p, c_int32,
+                                            c_void_p]
         L.oracle_bfs_multi.restype = c_int32
         L.oracle_bfs_multi.argtypes = [c_int64, c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
                                        c_void_p]
@@ -88,6 +91,17 @@ def bfs_queue(beg, adj, src, parents=False):
     par = np.empty(n, np.int32) if parents else None
     k = lib().oracle_bfs_queue(n, _p(beg), _p(adj), int(src), _p(lv), _p(par))
     return lv, par, k
+
+
+def bfs_queue_rows(beg, adj, srcs):
+    """oracle_bfs_queue from each source (rows computed in parallel threads).
+    Returns int32 levels [len(srcs), n]."""
+    beg, adj = _csr(beg, adj)
+    n = beg.shape[0] - 1
+    s = np.ascontiguousarray(srcs, np.int32)
+    out = np.empty((s.shape[0], n), np.int32)
+    lib().oracle_bfs_queue_rows(n, _p(beg), _p(adj), _p(s), s.shape[0], _p(out))
+    return out
 
 
 def bfs_multi(beg, adj, srcs, parents=False):

@@ -14,12 +14,12 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session", autouse=True)
 def _built_libraries():
-    """Builds the oracle and libxbfs when absent (never rebuilds existing ones)."""
+    """Builds the oracle when absent; brings libxbfs.so and ./bfs up to date
+    with their sources (make's dependency check: a library older than any
+    source is rebuilt, an up-to-date one is left alone)."""
     from oracle import oracle as O
     if not os.path.exists(O.LIB_PATH):
         O.build()
-    from xbfs_amd import _capi
-    if not os.path.exists(_capi.LIB_PATH):
-        from xbfs_amd import build as B
-        B.build(force=False)
+    from xbfs_amd import build as B
+    B.build(force=False)
     yield

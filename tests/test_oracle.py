@@ -176,6 +176,21 @@ def test_omp_td_matches_queue():
         assert (a == b).all() and t > 0
 
 
+def test_queue_rows_equal_single_source_rows():
+    """oracle_bfs_queue_rows (rows in parallel threads) = oracle_bfs_queue per
+    source, including a repeated source and a degree-0 source."""
+    from xbfs_amd import Graph
+    g = Graph.rmat(12, 8, seed=1212)
+    iso = int(np.flatnonzero(g.degrees() == 0)[0])
+    srcs = np.array([0, 5, 5, 100, iso, 4095], np.int32)
+    rows = O.bfs_queue_rows(g.beg_pos, g.adj, srcs)
+    assert rows.shape == (srcs.size, g.n)
+    for i, s in enumerate(srcs):
+        want, _, _ = O.bfs_queue(g.beg_pos, g.adj, int(s))
+        assert (rows[i] == want).all()
+    assert (rows[4] >= 0).sum() == 1
+
+
 def test_mlp_grad_finite_difference():
     rng = np.random.default_rng(0)
     P = (rng.standard_normal(O.MLP_NPARAM) * 0.3).astype(np.float32)

@@ -3,8 +3,10 @@
 The library is loaded from this package directory only; when it is missing
 every call raises, so nothing here can silently fall back to a CPU path.
 """
+import atexit
 import ctypes
 import os
+import weakref
 from ctypes import (POINTER, Structure, byref, c_char_p, c_double, c_float, c_int, c_int8,
                     c_int32, c_int64, c_uint64, c_void_p)
 
@@ -248,21 +250,45 @@ def default_opts(**kw):
     return o
 
 
+# Contexts still open when the interpreter exits are destroyed by an atexit
+# handler, which runs before module teardown and before the C runtime's own
+# exit handlers (a profiler's among them): device memory, streams and events
+# are then released while the HIP runtime is still fully up, never from a
+# finalizer during interpreter teardown.
+_live = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts():
+    for c in list(_live):
+        c.close()
+
+
 class Context:
     """Device context: the CSR uploaded to one GPU plus all traversal state."""
 
     def __init__(self, graph, device=0):
         self.graph = graph
+        self.h = None
         h = c_void_p()
         check(lib().xbfs_create(device, _ptr(graph.beg_pos), _ptr(graph.adj), graph.n, graph.m,
                                 byref(h)), "xbfs_create")
         self.h = h
         self.n, self.m = graph.n, graph.m
+        _live.add(self)
 
     def close(self):
         if self.h:
             lib().xbfs_destroy(self.h)
             self.h = None
+        _live.discard(self)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
     def __del__(self):
         try:
