@@ -186,6 +186,31 @@ int xbfs_bfs(xbfs_ctx *ctx, int32_t src, const xbfs_opts *opts,
              int32_t *level_out, int32_t *parent_out,
              xbfs_level_rec *rec, int32_t rec_cap, xbfs_stats *st);
 
+/* ---- allocation-free traversal -------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the traversal of xbfs_bfs on a CSR the caller already
+ * holds in device memory, with every buffer beyond the inputs and outputs in
+ * a library-owned device array of XBFS_NOALLOC_SCRATCH_BYTES per GPU (held
+ * from the library's module load until the process ends).  Nothing is
+ * allocated, host or device.
+ *   d_beg_pos int64[n+1], d_adj int32[m]: device pointers, n <= XBFS_NOALLOC_MAX_N
+ *     (larger n returns XBFS_ERR_LIMIT before touching the GPU);
+ *   d_level int32[n] (device, required), d_parent int32[n] (device, or NULL);
+ *   policy_weights: host pointer to XBFS_MLP_NPARAM floats for policy LEARNED,
+ *     or NULL (LEARNED then decides as HEURISTIC);
+ *   st (may be NULL): t_bfs_s is host wall time around the launch (no events:
+ *     creating one allocates).
+ * Rows are scanned in their stored order (no degree-ordered copy) and every
+ * traversal runs until its frontier is empty (no create-time component).
+ * Synchronous, on the null stream of `device`.  Calls that overlap on one GPU
+ * share its array and race: serialise them. */
+#define XBFS_NOALLOC_MAX_N (1 << 20)
+#define XBFS_NOALLOC_SCRATCH_BYTES (60u << 20)
+int xbfs_bfs_noalloc(int device, const int64_t *d_beg_pos, const int32_t *d_adj, int64_t n,
+                     int64_t m, int32_t src, const xbfs_opts *opts, const float *policy_weights,
+                     int32_t *d_level, int32_t *d_parent, xbfs_stats *st);
+
 /* ---- batched traversal (no reference counterpart) ------------------------ */
 
 /* Multi-source BFS: level[v] = hops from v to the nearest of the nsrc seeds

@@ -118,7 +118,11 @@ SIGNATURES = {
                                  POINTER(Stats)]),
     "xbfs_autotune": (c_int, [c_void_p, c_void_p, c_int32, c_int32, c_int32, POINTER(Opts),
                               POINTER(TuneRec), c_int32, POINTER(c_int32)]),
+    "xbfs_bfs_noalloc": (c_int, [c_int, c_void_p, c_void_p, c_int64, c_int64, c_int32,
+                                 POINTER(Opts), c_void_p, c_void_p, c_void_p, POINTER(Stats)]),
 }
+
+NOALLOC_MAX_N = 1 << 20
 
 _lib = None
 

@@ -1312,3 +1312,6 @@ extern "C" int xbfs_mlp_train(int device, float *params, const float *x, const f
         XB_HIP(hipMemcpy(loss_curve, curve.p, sizeof(float) * (size_t)steps, hipMemcpyDeviceToHost));
     return XBFS_OK;
 }
+
+// the allocation-free single-source entry point (its scratch array and set-up)
+#include "xbfs_noalloc.hip"
