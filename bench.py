@@ -18,14 +18,21 @@ per-level oracle-best schedule measured on a subset of the sources.
 XBFS_COMPONENT=off (every traversal expands levels until its frontier is
 empty instead of stopping once the source's component is covered).
 
-roofline: algorithmic bytes of one traversal, B = 8*m_cc + 20*V_cc + 4*V
-(int32 adjacency word + int32 level probe per traversed edge; int64 offset
-pair + int32 level write per reached vertex; level initialisation), divided
-by the device time of the launch that runs the traversal, against the
-8 TB/s HBM peak.
+roofline: the bytes one traversal's algorithm reads and writes as executed
+(direction-optimised: the work each level's mode actually did, from the level
+records: top-down levels expand m_f edges, bottom-up levels report the
+candidates they probed and the rows and adjacency words they scanned; byte
+costs per unit in traversal_bytes()), divided by the device time of the
+launch that runs the traversal, against the 8 TB/s HBM peak.  The byte count
+that charges every edge of the reached component, 8*m_cc + 20*V_cc + 4*V (the
+TEPS-equivalent figure), is reported beside it.  roofline.traffic is the HBM
+traffic per launch measured by rocprofv3 counters on this workload (the
+profile is named in traffic_source); it is reported only while the launch
+time stays within 10 % of the profiled one.
 
-cpu_baseline: OpenMP top-down BFS (the oracle's CPU baseline) on a bounded
-sample of the same sources, on this host's cores.  cpu_serial: the
+cpu_baseline: OpenMP top-down BFS (the oracle's CPU baseline) on the same
+sources, on every core this process may run on (its affinity set), for as
+many sources as fit a time budget (--cpu-budget seconds).  cpu_serial: the
 reference's own CPU path (the level-synchronous scan BFS of
 bfs_gpu_update.cu:104-127, restated in the oracle) on one core, first source.
 
@@ -59,12 +66,59 @@ HBM_PEAK = 8.0e12
 # HBM bytes per k_persistent launch on the default workload (RMAT-24 ef16, the
 # 64 seeded sources, learned policy), from two rocprofv3 --pmc passes
 # (profiles/k_persistent_rmat24_pmc_traffic.md): mean FETCH_SIZE and
-# WRITE_SIZE in KiB per dispatch over the 65 traversal launches (the context's
+# WRITE_SIZE in KiB per dispatch over the traversal launches (the context's
 # create-time traversal left out), each scaled by what the same counter reads
 # for a plain 1 GiB device copy (FETCH_SIZE reads half of the bytes a copy
-# moves: factor 2.0; WRITE_SIZE reads all of them: factor 1.0).
+# moves: factor 2.0; WRITE_SIZE reads all of them: factor 1.0).  kernel_us is
+# the mean launch time of the kernel trace taken with them; the figure is
+# reported only while the measured launch time stays within 10 % of it.
 PMC_TRAFFIC = {"workload": (24, 16, 64, "learned"), "fetch_kib": 409197.8, "fetch_factor": 1.99995,
-               "write_kib": 256612.2, "write_factor": 1.0}
+               "write_kib": 256612.2, "write_factor": 1.0, "kernel_us": 418.2,
+               "profile": "profiles/k_persistent_rmat24_pmc_traffic.md"}
+
+MODE_SF, MODE_SS, MODE_BU = 0, 1, 2
+
+
+def traversal_bytes(recs, n, v_cc, parents=False):
+    """Bytes one traversal reads and writes as executed, from its level
+    records.  W = one bitmap (n/8 bytes, rounded up to 64-bit words).
+      initialisation: 4n level words (+4n parents), visited + 4 frontier rings (5W)
+      every level: one frontier ring zeroed (W)
+      sweep of F_L into the binned queue (a top-down level whose queue is not
+        exact): W of bitmap, 16 B offsets + 4 B queue entry per frontier vertex;
+        after a single-scan level the sweep reads the 4n level words instead
+        and writes F_L and the visited words (2W)
+      top-down: 16 B offsets per frontier vertex, 8 B per expanded edge (4 B
+        adjacency word + 4 B level probe), 4 B level write per discovered
+        vertex; scan-free also 16 B of bitmap atomics, 16 B offsets and a 4 B
+        queue entry per discovered vertex
+      bottom-up: candidate words (mask and visited, 2W), 12 B per candidate
+        probed (8 B hub record + 4 B bitmap word), 16 B offsets per scanned row,
+        8 B per scanned adjacency word (word + bitmap probe), 4 B level write
+        per discovered vertex, visited and next-frontier words written (2W)
+      parents: 4 B per discovered vertex."""
+    W = 8 * ((n + 63) // 64)
+    b = 4 * n + 5 * W + (4 * n if parents else 0)
+    prev = -1
+    seen = sum(r["n_f"] for r in recs)
+    for i, r in enumerate(recs):
+        mode, nf, mf = r["mode"], r["n_f"], r["m_f"]
+        nd = recs[i + 1]["n_f"] if i + 1 < len(recs) else max(v_cc - seen, 0)
+        b += W
+        if prev == MODE_SS:
+            b += 4 * n + 2 * W + 20 * nf
+        elif mode != MODE_BU and i > 0 and prev != MODE_SF:
+            b += W + 20 * nf
+        if mode == MODE_BU:
+            b += 4 * W + 12 * r["bu_cand"] + 16 * r["bu_scan_v"] + 8 * r["bu_scan_e"] + 4 * nd
+        else:
+            b += 16 * nf + 8 * mf + 4 * nd
+            if mode == MODE_SF:
+                b += 36 * nd
+        if parents:
+            b += 4 * nd
+        prev = mode
+    return b
 
 
 def hmean(x):
@@ -151,9 +205,14 @@ def main():
     ap.add_argument("--sources", type=int, default=64)
     ap.add_argument("--persistent", type=int, default=-1,
                     help="1 = device-side level loop, 0 = host-driven, -1 = library default")
-    ap.add_argument("--cpu-sample", type=int, default=4,
-                    help="sources timed for the CPU baseline (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-sample", type=int, default=64,
+                    help="sources timed for the CPU baseline at most (0 = skip)")
+    ap.add_argument("--cpu-budget", type=float, default=20.0,
+                    help="seconds of CPU baseline work: sources are timed in order until "
+                         "the budget is spent (at least one)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="OpenMP threads of the CPU baseline (0 = every core this process "
+                         "may run on)")
     ap.add_argument("--cpu-serial", type=int, default=1,
                     help="sources timed for the single-core serial reference path (0 = skip)")
     ap.add_argument("--oracle-best", type=int, default=16,
@@ -202,7 +261,9 @@ def main():
     t_gen = time.time() - t0
     deg = g.degrees()
     srcs = pick_sources(deg, args.sources)
+    t_c0 = time.perf_counter()
     ctx = Context(g, device=0)
+    t_create = time.perf_counter() - t_c0
     kw = {}
     if args.persistent >= 0:
         kw["persistent"] = args.persistent
@@ -247,12 +308,29 @@ def main():
     main_pol = pols[-1][0]
     gteps = m_cc / t_dev[main_pol] / 1e9
     value = hmean(gteps)
-    b_alg = 8 * m_cc + 20 * v_cc + 4 * g.n
-    achieved = float(b_alg.sum() / t_dev[main_pol].sum())
-    traffic = None
+    # bytes as executed, from each traversal's level records (one untimed
+    # pass; the decisions and the work counts do not depend on timing)
+    b_do = np.zeros(len(srcs))
+    lvl_bytes = {}
+    for i, s in enumerate(srcs):
+        _, _, recs, st1 = ctx.bfs(int(s), opts=opts, want_levels=False, rec_cap=256)
+        b_do[i] = traversal_bytes(recs, g.n, st1["v_cc"])
+        for r in recs:
+            key = (r["level"], r["mode"])
+            one = traversal_bytes([r], g.n, 0) - traversal_bytes([], g.n, 0)
+            lvl_bytes[key] = lvl_bytes.get(key, 0) + one
+    b_teps = 8 * m_cc + 20 * v_cc + 4 * g.n
+    t_mean = float(t_dev[main_pol].mean())
+    achieved = float(b_do.mean()) / t_mean
+    traffic, traffic_note = None, None
     if args.graph == "rmat" and PMC_TRAFFIC["workload"] == (args.scale, args.ef, len(srcs), main_pol):
-        traffic = 1024.0 * (PMC_TRAFFIC["fetch_kib"] * PMC_TRAFFIC["fetch_factor"] +
-                            PMC_TRAFFIC["write_kib"] * PMC_TRAFFIC["write_factor"])
+        drift = abs(t_mean * 1e6 - PMC_TRAFFIC["kernel_us"]) / PMC_TRAFFIC["kernel_us"]
+        if drift <= 0.10:
+            traffic = 1024.0 * (PMC_TRAFFIC["fetch_kib"] * PMC_TRAFFIC["fetch_factor"] +
+                                PMC_TRAFFIC["write_kib"] * PMC_TRAFFIC["write_factor"])
+        else:
+            traffic_note = (f"stale: launch time {t_mean * 1e6:.1f} us differs from the profiled "
+                            f"{PMC_TRAFFIC['kernel_us']} us by {drift * 100:.0f} %")
     policy_cmp = {}
     if "heuristic" in t_dev:
         policy_cmp["heuristic_gteps_hmean"] = round(hmean(m_cc / t_dev["heuristic"] / 1e9), 3)
@@ -305,13 +383,20 @@ def main():
 
     cpu = None
     if args.cpu_sample > 0:
-        nthr = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        cg = []
+        usable = len(os.sched_getaffinity(0))
+        nthr = args.cpu_threads or usable
+        cg, spent = [], 0.0
         for s in srcs[: args.cpu_sample]:
             lvc, tc = O.omp_td_bfs(g.beg_pos, g.adj, int(s), nthr)
             cg.append(float(deg[lvc >= 0].sum()) / tc / 1e9)
+            spent += tc
+            if spent >= args.cpu_budget:
+                break
         cpu = {"value": hmean(cg), "unit": "GTEPS", "cores": nthr, "kind": "port",
-               "sample": f"OpenMP top-down BFS, first {len(cg)} of the {len(srcs)} sources, same graph"}
+               "sample": f"OpenMP top-down BFS (frontier queue, CAS on levels), first {len(cg)} "
+                         f"of the {len(srcs)} sources ({spent:.1f} s of traversal, budget "
+                         f"{args.cpu_budget:g} s), same graph",
+               "host_cpus": os.cpu_count(), "affinity_cpus": usable}
     cpu_serial = None
     if args.cpu_serial > 0:
         cs = []
@@ -350,18 +435,36 @@ def main():
                    "n": g.n, "m": g.m, "persistent": int(opts.persistent),
                    "policy": ("learned MLP cost model (heuristic fallback)" if have_model
                               else "heuristic (reference alpha rule, alpha=0.1)"),
-                   "levels_valid": bool(valid)},
+                   "levels_valid": bool(valid),
+                   "preprocessing": {
+                       "context_create_s": round(t_create, 3),
+                       "note": "once per graph, outside the timed region: upload, the "
+                               "degree-ordered bottom-up copy of the adjacency, the hub table, "
+                               "and one traversal from the largest-degree vertex whose "
+                               "component every traversal from a member then uses to stop "
+                               "early and to skip other components' bottom-up candidates; "
+                               "without_component_end is the same run without that component"}},
         "policies": policy_cmp,
         "without_component_end": no_comp,
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
                      "traffic": None if traffic is None else round(traffic),
-                     "algorithmic_bytes_per_launch": round(float(b_alg.mean())),
-                     "measured_hbm_gbs": (None if traffic is None else
-                                          round(traffic / float(t_dev[main_pol].mean()) / 1e9, 1)),
+                     "algorithmic_bytes_per_launch": round(float(b_do.mean())),
+                     "bytes_model": "as executed, per level from the level records "
+                                    "(bench.traversal_bytes)",
+                     "bytes_per_level_mode": {f"L{k[0]}_{['td_sf', 'td_ss', 'bu'][k[1]]}":
+                                              round(v / len(srcs)) for k, v in sorted(lvl_bytes.items())},
+                     "teps_equivalent": {"bytes_per_launch": round(float(b_teps.mean())),
+                                         "gbs": round(float(b_teps.mean()) / t_mean / 1e9, 2),
+                                         "model": "8*m_cc + 20*V_cc + 4*V (every edge of the "
+                                                  "reached component charged)"},
+                     "measured_hbm_gbs": (None if traffic is None else round(traffic / t_mean / 1e9, 1)),
+                     "traffic_source": (f"{PMC_TRAFFIC['profile']} (mean launch "
+                                        f"{PMC_TRAFFIC['kernel_us']} us when profiled)"
+                                        if traffic is not None else traffic_note),
                      "kernel": "k_persistent" if opts.persistent else "host-driven level loop",
                      "launches_per_bfs": 1 if opts.persistent else None,
-                     "avg_launch_us": round(float(t_dev[main_pol].mean()) * 1e6, 2)},
+                     "avg_launch_us": round(t_mean * 1e6, 2)},
         "cpu_baseline": cpu,
         "cpu_serial": cpu_serial,
         "batch64": b64,

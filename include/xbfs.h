@@ -168,6 +168,11 @@ typedef struct {
     float ratio;     /* (float)m_f / (float)E                                 */
     float pad;
     double t_s;      /* device seconds for this level (when time_levels)      */
+    /* work a bottom-up level executed (0 for top-down levels): candidates
+     * probed at their largest-degree neighbour, candidates whose rows were
+     * then scanned, adjacency words those scans read.  A top-down level
+     * expands exactly m_f edges. */
+    int64_t bu_cand, bu_scan_v, bu_scan_e;
 } xbfs_level_rec;
 
 typedef struct {

@@ -126,11 +126,17 @@ def test_per_level_records_csv(tmp_path):
         assert p.returncode == 0, p.stdout + p.stderr
     rows = list(csv.DictReader(open(out)))
     assert list(rows[0].keys()) == ["dataset", "src", "level", "mode", "tag", "n_f", "m_f", "n_u",
-                                    "m_u", "ratio", "t_s"]
+                                    "m_u", "ratio", "t_s", "bu_cand", "bu_scan_v", "bu_scan_e"]
     want, _, _ = O.bfs_queue(g.beg_pos, g.adj, src)
     cnt, nv, _ = O.level_counts(g.beg_pos, want)
+    deg = g.degrees()
+    # bottom-up candidates at level L: the unvisited vertices of the source's
+    # component when it is the largest-degree vertex's, else every unvisited
+    # non-isolated vertex
+    hub_comp = O.bfs_queue(g.beg_pos, g.adj, int(np.argmax(deg)))[0] >= 0
     runs = [r for r in rows if r["level"] == "0"]
     assert len(runs) == 2
+    n_bu = 0
     for r in rows:
         L = int(r["level"])
         assert r["dataset"] == "rmat14" and int(r["src"]) == src
@@ -138,3 +144,12 @@ def test_per_level_records_csv(tmp_path):
         assert int(r["tag"]) == {0: 0, 1: 1, 2: -1}[int(r["mode"])]
         assert abs(float(r["ratio"]) - cnt[L] / g.m) < 1e-6
         assert float(r["t_s"]) > 0
+        cand, sv, se = int(r["bu_cand"]), int(r["bu_scan_v"]), int(r["bu_scan_e"])
+        if int(r["mode"]) == 2:
+            n_bu += 1
+            unv = (want > L) | ((want == -1) & (deg > 0) & ~hub_comp[src])
+            assert cand == unv.sum()
+            assert 0 <= sv <= cand and 0 <= se <= deg[unv].sum()
+        else:
+            assert cand == sv == se == 0
+    assert n_bu > 0

@@ -56,7 +56,8 @@ class Opts(Structure):
 class LevelRec(Structure):
     _fields_ = [("level", c_int32), ("mode", c_int32), ("tag", c_int32), ("policy", c_int32),
                 ("n_f", c_int64), ("m_f", c_int64), ("n_u", c_int64), ("m_u", c_int64),
-                ("ratio", c_float), ("pad", c_float), ("t_s", c_double)]
+                ("ratio", c_float), ("pad", c_float), ("t_s", c_double),
+                ("bu_cand", c_int64), ("bu_scan_v", c_int64), ("bu_scan_e", c_int64)]
 
 
 class Stats(Structure):

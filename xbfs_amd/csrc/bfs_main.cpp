@@ -207,12 +207,15 @@ int main(int argc, char **argv)
         FILE *f = fopen(p, "a+");
         if (f) {
             fseek(f, 0, SEEK_END);
-            if (ftell(f) == 0) fprintf(f, "dataset,src,level,mode,tag,n_f,m_f,n_u,m_u,ratio,t_s\n");
+            if (ftell(f) == 0)
+                fprintf(f, "dataset,src,level,mode,tag,n_f,m_f,n_u,m_u,ratio,t_s,bu_cand,bu_scan_v,"
+                           "bu_scan_e\n");
             for (int L = 0; L < st.n_levels && L < (int)rec.size(); L++) {
                 const xbfs_level_rec &r = rec[(size_t)L];
-                fprintf(f, "%s,%d,%d,%d,%d,%lld,%lld,%lld,%lld,%.9g,%.9g\n", dataset.c_str(), src,
-                        r.level, r.mode, r.tag, (long long)r.n_f, (long long)r.m_f, (long long)r.n_u,
-                        (long long)r.m_u, (double)r.ratio, r.t_s);
+                fprintf(f, "%s,%d,%d,%d,%d,%lld,%lld,%lld,%lld,%.9g,%.9g,%lld,%lld,%lld\n",
+                        dataset.c_str(), src, r.level, r.mode, r.tag, (long long)r.n_f,
+                        (long long)r.m_f, (long long)r.n_u, (long long)r.m_u, (double)r.ratio, r.t_s,
+                        (long long)r.bu_cand, (long long)r.bu_scan_v, (long long)r.bu_scan_e);
             }
             fclose(f);
         }

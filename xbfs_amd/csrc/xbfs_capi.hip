@@ -374,6 +374,7 @@ static void fill_rec(xbfs_level_rec &r, int32_t L, int32_t mode, int32_t htag, i
     r.ratio = (float)m_f / (float)E;
     r.pad = (float)htag;
     r.t_s = t_s;
+    r.bu_cand = r.bu_scan_v = r.bu_scan_e = 0;
 }
 
 // Enqueues the kernels of level L in `mode` on the context stream: zeroing
@@ -444,6 +445,11 @@ static int bfs_host_loop(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs_
         pend_n = h.sum[xb_f_ring(L + 1, 0)];
         pend_m = h.sum[xb_f_ring(L + 1, 1)];
         const long long p2_next = h.sum[xb_f_ring(L + 1, 2)];
+        if (rec && L > 0 && L - 1 < rec_cap) {
+            rec[L - 1].bu_cand = h.sum[xb_f_bu(L - 1, 0)];
+            rec[L - 1].bu_scan_v = h.sum[xb_f_bu(L - 1, 1)];
+            rec[L - 1].bu_scan_e = h.sum[xb_f_bu(L - 1, 2)];
+        }
         long long n_u = 0, m_u = 0;
         if (!xb_level_open(s, a, n_f, m_f, pend_n, pend_m, &n_u, &m_u)) break;
         if (L >= a.max_levels) return XBFS_ERR_LIMIT;
@@ -528,9 +534,13 @@ static int bfs_persistent(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs
             const double t_s = (double)(d.t1 - d.t0) / c->wall_hz;
             if (d.mode == XBFS_MODE_BU) bu_s += t_s;
             else td_s += t_s;
-            if (rec && L < rec_cap)
+            if (rec && L < rec_cap) {
                 fill_rec(rec[L], L, d.mode, (int8_t)(d.htag & 0xff), (d.htag >> 8) & 0xff, d.n_f, d.m_f,
                          d.n_u, d.m_u, c->m, t_s);
+                rec[L].bu_cand = d.bu_cand;
+                rec[L].bu_scan_v = d.bu_scan_v;
+                rec[L].bu_scan_e = d.bu_scan_e;
+            }
         }
     }
     if (st) {

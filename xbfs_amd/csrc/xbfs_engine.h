@@ -52,7 +52,7 @@ struct alignas(256) XbCtl {
     long long v_cc, m_cc;
     long long seed_iso;                 // multi-source: distinct degree-0 seeds (level 0, no edges)
     long long pad2;
-    long long sum[16];                  // host-driven loop: reduced statistics (k_part_reduce)
+    long long sum[32];                  // host-driven loop: reduced statistics (k_part_reduce)
 };
 
 // Per-level statistics live in per-block partial sums (no atomics): block b
@@ -60,17 +60,23 @@ struct alignas(256) XbCtl {
 // touches column b within a phase) and readers sum the column after the
 // barrier.  Fields: ring slot r = level & 3 -> f = 3r + {0: vertices
 // discovered for that level, 1: their degree sum, 2: of which by the +2 rule};
-// sweep slot s = level & 1 -> f = 12 + 2s + {0: count, 1: degree sum}.
+// sweep slot s = level & 1 -> f = 12 + 2s + {0: count, 1: degree sum};
+// work of a bottom-up level L, kept in ring slot r = (L + 1) & 3 (the level
+// it discovers, zeroed on the same schedule) -> f = 16 + 3r + {0: candidates
+// probed at their hub, 1: candidates whose rows were scanned, 2: adjacency
+// words those scans read}.
 #define XB_PMAX 4096
-#define XB_NFIELDS 16
+#define XB_NFIELDS 28
 __host__ __device__ inline int xb_f_ring(int32_t level, int k) { return 3 * (level & 3) + k; }
 __host__ __device__ inline int xb_f_sweep(int32_t level, int k) { return 12 + 2 * (level & 1) + k; }
+__host__ __device__ inline int xb_f_bu(int32_t level, int k) { return 16 + 3 * ((level + 1) & 3) + k; }
 
 // Per-level record written on the device by the persistent kernel.
 struct XbDevRec {
     int32_t level, mode, tag, htag;
     long long n_f, m_f, n_u, m_u;
     long long t0, t1; // wall clock ticks
+    long long bu_cand, bu_scan_v, bu_scan_e; // bottom-up work (xb_f_bu fields), 0 for top-down
 };
 
 struct XbArgs {

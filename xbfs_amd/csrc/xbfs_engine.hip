@@ -93,16 +93,17 @@ __device__ void xb_part_add(const XbArgs &a, int bid, unsigned long long (&v)[K]
 
 // Sums of K fields over the first ncols columns of the partial statistics
 // (the blocks that wrote them; the zeroing schedule clears every column, so
-// the others are zero), computed by wave 0 alone: every lane issues all its
-// loads at once, then one wave reduction.  Lane 0 of wave 0 gets the
-// results; other threads must not use out.  The columns are read 256 at a
-// time (4 per lane and field, unrolled), so up to 4K loads per lane are in
-// flight before the first wait: one memory round trip per 256 blocks instead
-// of one per column.
+// the others are zero), computed by wave `wave` of the block alone: every
+// lane issues all its loads at once, then one wave reduction.  Lane 0 of that
+// wave gets the results; other threads must not use out.  The columns are
+// read 256 at a time (4 per lane and field, unrolled), so up to 4K loads per
+// lane are in flight before the first wait: one memory round trip per 256
+// blocks instead of one per column.
 template <int K>
-__device__ void xb_part_sum(const XbArgs &a, const int (&f)[K], long long (&out)[K], int ncols)
+__device__ void xb_part_sum(const XbArgs &a, const int (&f)[K], long long (&out)[K], int ncols,
+                            int wave = 0)
 {
-    if (threadIdx.x >= 64) return;
+    if ((int)(threadIdx.x >> 6) != wave) return;
     const int ln = xb_lane();
     long long acc[K];
 #pragma unroll
@@ -325,9 +326,11 @@ __device__ void ph_level_prep(const XbArgs &a, int32_t L, int bid, int nb, int t
     }
     const int64_t gtid = (int64_t)bid * bdim + tid;
     const int64_t nthr = (int64_t)nb * bdim;
-    const int zf[5] = {xb_f_ring(L + 3, 0), xb_f_ring(L + 3, 1), xb_f_ring(L + 3, 2),
-                       xb_f_sweep(L + 1, 0), xb_f_sweep(L + 1, 1)};
-    for (int64_t i = gtid; i < 5 * XB_PMAX; i += nthr)
+    // ring slot (L+3)&3 holds the bottom-up work of level L+2 as well
+    const int zf[8] = {xb_f_ring(L + 3, 0), xb_f_ring(L + 3, 1), xb_f_ring(L + 3, 2),
+                       xb_f_sweep(L + 1, 0), xb_f_sweep(L + 1, 1),
+                       xb_f_bu(L + 2, 0),    xb_f_bu(L + 2, 1),    xb_f_bu(L + 2, 2)};
+    for (int64_t i = gtid; i < 8 * XB_PMAX; i += nthr)
         a.part[(int64_t)zf[i / XB_PMAX] * XB_PMAX + (i % XB_PMAX)] = 0;
     uint64_t *F3 = xb_f(a, L + 3);
     for (int64_t w = gtid; w < a.nwords; w += nthr) F3[w] = 0ull;
@@ -724,6 +727,9 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
     const int64_t nchunks = (a.nwords + XB_CHUNK_WORDS - 1) / XB_CHUNK_WORDS;
     const uint64_t below = ln ? (~0ull >> (64 - ln)) : 0ull;
     unsigned long long n1 = 0, m1 = 0;
+    // work counters: candidates probed and rows scanned (lane 0, wave totals),
+    // adjacency words read by this lane's scans
+    uint32_t w_cand = 0, w_scanv = 0, w_scane = 0;
     // The wave's chunks gw, gw + W, gw + 2W, ... in groups of 4: lane 16q + k
     // loads word k of the group's chunk q, so one memory round trip covers 4
     // chunks and empty chunks are skipped by one ballot.
@@ -747,6 +753,7 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
         const int inc = xb_wave_incl_scan_i32(pc);
         const int total = __shfl(inc, 63);
         if (total == 0) continue;
+        if (ln == 0) w_cand += (uint32_t)total;
         const int off = inc - pc;
         // the chunk's candidates in ascending order
         for (int k = 0; k < XB_CHUNK_WORDS; k++) {
@@ -823,6 +830,7 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
         // dependent round trip per step of every full scan, more than it
         // saves at the next level.)
         const int64_t skip = a.hub_first ? 1 : 0;
+        if (ln == 0) w_scanv += (uint32_t)nres;
         int ndef = 0, next = 0;
         int32_t v = -1;
         int64_t j = 0, ce = 0, e = 0, b = 0;
@@ -852,6 +860,7 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                 const bool h1 = u1 >= 0 && ((F32[u1 >> 5] >> (u1 & 31)) & 1u);
                 const bool h2 = u2 >= 0 && ((F32[u2 >> 5] >> (u2 & 31)) & 1u);
                 const bool h3 = u3 >= 0 && ((F32[u3 >> 5] >> (u3 & 31)) & 1u);
+                w_scane += 1u + (u1 >= 0) + (u2 >= 0) + (u3 >= 0);
                 const unsigned long long bit = 1ull << (v & 63);
                 const int li = (int)((v >> 6) - w0);
                 if (h0 | h1 | h2 | h3) {
@@ -895,7 +904,10 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                 }
                 bool h[4];
 #pragma unroll
-                for (int t = 0; t < 4; t++) h[t] = u[t] >= 0 && ((F32[u[t] >> 5] >> (u[t] & 31)) & 1u);
+                for (int t = 0; t < 4; t++) {
+                    h[t] = u[t] >= 0 && ((F32[u[t] >> 5] >> (u[t] & 31)) & 1u);
+                    w_scane += u[t] >= 0;
+                }
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
                     const unsigned long long m = __ballot(h[t]);
@@ -930,10 +942,13 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
         }
     }
     }
-    unsigned long long vv[2] = {n1, m1};
-    const int ff[2] = {xb_f_ring(L1, 0), xb_f_ring(L1, 1)};
-    long long old[2] = {carry ? carry[0] : 0, carry ? carry[1] : 0}, tot[2];
-    xb_part_add<2>(a, bid, vv, ff, carry != nullptr, old, tot);
+    // the work fields of this level start at zero (zeroing schedule) and only
+    // this phase writes them
+    unsigned long long vv[5] = {n1, m1, w_cand, w_scanv, w_scane};
+    const int ff[5] = {xb_f_ring(L1, 0), xb_f_ring(L1, 1), xb_f_bu(L, 0), xb_f_bu(L, 1),
+                       xb_f_bu(L, 2)};
+    long long old[5] = {carry ? carry[0] : 0, carry ? carry[1] : 0, 0, 0, 0}, tot[5];
+    xb_part_add<5>(a, bid, vv, ff, carry != nullptr, old, tot);
     if (carry && threadIdx.x == 0) {
         carry[0] = 0;
         carry[1] = 0;
@@ -1045,7 +1060,22 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
         // other waves zero what level L+1 will write; one block barrier
         // publishes the decision.  Blocks that sit out small-frontier levels
         // and rejoin later skip this level's zeroing: the active blocks did it.
-        if (threadIdx.x >= 64) ph_level_prep(a, L, bid, nact, 64);
+        // Wave 1 of block 0 meanwhile sums the previous level's bottom-up work
+        // into its record (zeros after a top-down level).
+        if (threadIdx.x >= 128) ph_level_prep(a, L, bid, nact, 128);
+        if (threadIdx.x >= 64 && threadIdx.x < 128 && bid == 0 && L > 0 && L - 1 < a.rec_cap) {
+            long long wv[3] = {0, 0, 0};
+            if (ls.st.prev_mode == XBFS_MODE_BU) {
+                const int fw[3] = {xb_f_bu(L - 1, 0), xb_f_bu(L - 1, 1), xb_f_bu(L - 1, 2)};
+                xb_part_sum<3>(a, fw, wv, nact, 1);
+            }
+            if (threadIdx.x == 64) {
+                XbDevRec &pr = a.recs[L - 1];
+                pr.bu_cand = wv[0];
+                pr.bu_scan_v = wv[1];
+                pr.bu_scan_e = wv[2];
+            }
+        }
         if (threadIdx.x < 64) {
             long long sv[5];
             const int ff[5] = {need_sweep ? xb_f_sweep(L, 0) : xb_f_ring(L, 0),
