@@ -37,12 +37,14 @@ reference's own CPU path (the level-synchronous scan BFS of
 bfs_gpu_update.cu:104-127, restated in the oracle) on one core, first source.
 
 batch64 (reported beside the headline, --batch 1): RMAT scale 22 ef16 (seed
-1022), 64 seeded sources; device time of one 64-source bit-parallel batch
-against the sum of the 64 single-source traversals of the same sources
-(medians of 3).  multigraph: 256 RMAT graphs (scale U{10..16}, edge factor
-U{4..32}, seeds 10000+i), one seeded source each; one multi-source traversal
-of the packed graphs against 256 single-source traversals on per-graph
-contexts, with the autotuner's setting for the packed graph.
+1022), 64 seeded sources; device time of one 64-source bit-parallel batch (one
+launch) against the sum of the 64 single-source traversals of the same
+sources under the learned policy, the fastest single-source path (medians of
+3; the heuristic's sum beside it).  multigraph: 256 RMAT graphs (scale
+U{10..16}, edge factor U{4..32}, seeds 10000+i), one seeded source each; one
+multi-source traversal of the packed graphs against 256 single-source
+traversals on per-graph contexts, likewise under both policies, with the
+autotuner's setting for the packed graph.
 
 Other graphs, parity and figures only, never the headline: --graph er
 (Erdos-Renyi, 2^scale vertices, 2^scale * ef / 2 edges) and --graph grid
@@ -133,25 +135,46 @@ def pick_sources(deg, k, seed=42):
 
 
 def batch_fields(reps=3):
-    """The two batched workloads; returns (batch64 dict, multigraph dict)."""
-    from xbfs_amd import Context, Graph
+    """The two batched workloads; returns (batch64 dict, multigraph dict).
+    Each batch is compared with one-at-a-time traversals of the same sources
+    under the fastest single-source path (the learned policy when its weights
+    are present) and, beside it, under the reference heuristic."""
+    from xbfs_amd import Context, Graph, default_opts, POLICY_HEURISTIC, POLICY_LEARNED
+    from xbfs_amd import policy as P
+    w = P.load() if os.path.exists(P.WEIGHTS) else None
+    pols = [("heuristic", default_opts(policy=POLICY_HEURISTIC))]
+    if w is not None:
+        pols.append(("learned", default_opts(policy=POLICY_LEARNED)))
+    best = pols[-1][0]
+
+    def seq_times(ctx, srcs):
+        out = {}
+        for name, o in pols:
+            out[name] = sum(float(np.median([ctx.bfs(int(s), opts=o, want_levels=False)[3]["t_bfs_s"]
+                                             for _ in range(reps)])) for s in srcs)
+        return out
+
     g = Graph.rmat(22, 16, 0.57, 0.19, 0.19, seed=1022)
     ctx = Context(g, device=0)
+    if w is not None:
+        ctx.load_policy(w)
     srcs = pick_sources(g.degrees(), 64)
     ctx.bfs_batch64(srcs, want_levels=False)
-    seq = 0.0
     for s in srcs:
-        seq += float(np.median([ctx.bfs(int(s), want_levels=False)[3]["t_bfs_s"]
-                                for _ in range(reps)]))
+        ctx.bfs(int(s), opts=pols[-1][1], want_levels=False)
+    seq = seq_times(ctx, srcs)
     tb, st = [], None
     for _ in range(reps):
         _, st = ctx.bfs_batch64(srcs, want_levels=False)
         tb.append(st["t_bfs_s"])
     tb = float(np.median(tb))
     b64 = {"workload": "RMAT scale 22 ef16 seed 1022, 64 seeded sources", "n": g.n, "m": g.m,
-           "batch_ms": round(tb * 1e3, 3), "sequential_ms": round(seq * 1e3, 3),
-           "speedup": round(seq / tb, 3), "aggregate_gteps": round(st["m_cc"] / tb / 1e9, 1),
-           "levels": st["n_levels"]}
+           "batch_ms": round(tb * 1e3, 3), "sequential_ms": round(seq[best] * 1e3, 3),
+           "sequential_policy": best, "speedup": round(seq[best] / tb, 3),
+           "sequential_heuristic_ms": round(seq["heuristic"] * 1e3, 3),
+           "speedup_vs_heuristic": round(seq["heuristic"] / tb, 3),
+           "aggregate_gteps": round(st["m_cc"] / tb / 1e9, 1), "levels": st["n_levels"],
+           "launches_per_batch": 1}
     ctx.close()
     del g, ctx
 
@@ -163,11 +186,14 @@ def batch_fields(reps=3):
         gi = Graph.rmat(scale, ef, 0.57, 0.19, 0.19, seed=10000 + i)
         gs.append(gi)
         ls.append(int(pick_sources(gi.degrees(), 1, seed=10000 + i)[0]))
-    seq = 0.0
+    seq = {name: 0.0 for name, _ in pols}
     for gi, s in zip(gs, ls):
         ci = Context(gi, device=0)
+        if w is not None:
+            ci.load_policy(w)
         ci.bfs(s, want_levels=False)
-        seq += float(np.median([ci.bfs(s, want_levels=False)[3]["t_bfs_s"] for _ in range(reps)]))
+        for name, t in seq_times(ci, [s]).items():
+            seq[name] += t
         ci.close()
     u, off = Graph.concat(gs)
     cu = Context(u, device=0)
@@ -180,8 +206,10 @@ def batch_fields(reps=3):
     tm = float(np.median(tm))
     t_default = trials[0]["t_s"]
     mg = {"workload": "256 RMAT graphs, scale U{10..16}, ef U{4..32}, seeds 10000+i, one source each",
-          "n": u.n, "m": u.m, "batch_ms": round(tm * 1e3, 3), "sequential_ms": round(seq * 1e3, 3),
-          "speedup": round(seq / tm, 3), "aggregate_gteps": round(st["m_cc"] / tm / 1e9, 1),
+          "n": u.n, "m": u.m, "batch_ms": round(tm * 1e3, 3), "sequential_ms": round(seq[best] * 1e3, 3),
+          "sequential_policy": best, "speedup": round(seq[best] / tm, 3),
+          "sequential_heuristic_ms": round(seq["heuristic"] * 1e3, 3),
+          "aggregate_gteps": round(st["m_cc"] / tm / 1e9, 1),
           "levels": st["n_levels"], "autotune": {
               "bin_thread_max": tuned.bin_thread_max, "bin_wave_max": tuned.bin_wave_max,
               "bin_block_max": tuned.bin_block_max, "grid_dim": tuned.grid_dim,

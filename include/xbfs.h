@@ -243,10 +243,16 @@ int xbfs_bfs_multigraph(xbfs_ctx *ctx, const int64_t *vert_off, int32_t k,
 
 /* 64-source batch: up to 64 single-source BFSs at once, source s as bit s of
  * a 64-bit frontier / visited mask per vertex, expanded top-down (push) or
- * bottom-up (pull) per level by the union frontier's size.
+ * bottom-up (pull) per level by the union frontier's size, in one cooperative
+ * launch over a copy of the graph relabelled by degree (built on the first
+ * call, about 12 bytes per edge and 40 per vertex).  The graph is taken as
+ * symmetric (bottom-up reads a row as its vertex's in-neighbours), as the
+ * single-source bottom-up does.
  * levels_out: host int32[nsrc * n], row s = levels from srcs[s], or NULL
  * (rows stay on the device).  st (may be NULL): t_bfs_s = device time of the
- * whole batch, v_cc / m_cc = sums over the sources, n_levels = levels run. */
+ * whole batch, v_cc / m_cc = sums over the sources, n_levels = levels run,
+ * td_s / bu_s = device time of its push / pull levels, gen_s = of the level
+ * rows. */
 int xbfs_bfs_batch64(xbfs_ctx *ctx, const int32_t *srcs, int32_t nsrc, const xbfs_opts *opts,
                      int32_t *levels_out, xbfs_stats *st);
 

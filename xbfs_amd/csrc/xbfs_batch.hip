@@ -1,335 +1,534 @@
 // 64-source batched BFS (no reference counterpart; the reference's README.md:80
 // cites iBFS-style concurrent traversals).  Up to 64 single-source BFSs run as
-// one traversal: source s is bit s of a 64-bit mask per vertex.
+// one traversal: source s is bit s of a 64-bit mask per vertex, and the whole
+// batch -- clearing, seeding, every level, the per-source level rows -- is one
+// cooperative launch of k_b64_persistent.
 //
-// State (n vertices, m directed edges, H history slots):
-//   F[2]   uint64[n]   frontier masks, ping-pong by level parity:
-//                      bit s of F_L[v] <=> level_s(v) == L
-//   acc    uint64[n]   next-level accumulator (atomic OR only)
-//   hist   uint64[H*n] visited history, slot L = vis_L: bit s of
-//                      hist[L][v] <=> level_s(v) <= L (monotone in L)
-//   lv     int32[64*n] per-source levels, row s = levels from source s
-//   seg    32-edge segments of the adjacency (seg_u = owning vertex, seg_e =
-//          first edge; segments of consecutive vertices are contiguous, so
-//          segment i ends at seg_e[i+1]): the unit of edge work, which bounds
-//          any lane's loop at 32 edges whatever the degree skew.
+// Vertex order.  The batch works on a relabelled copy of the graph: rank r is
+// the r-th vertex by degree, descending (ties by id), and every row lists its
+// neighbours by rank, ascending (largest degree first).  On skewed graphs most
+// edge endpoints are a few hubs, so the frontier masks a bottom-up level
+// gathers concentrate in the first few megabytes of the mask array, which
+// stay in each XCD's L2, and a row's scan meets the likeliest frontier
+// members first.  Vertices of similar degree are neighbours in rank order, so
+// a wave's lanes get rows of similar length, and the isolated vertices form
+// one tail that no level touches.
 //
-// Level L (expands F_L into level L+1):
-//   push  (top-down)  lane per segment of a frontier vertex u: for each
-//                     neighbour v, acc[v] |= F_L[u] & ~vis_L[v]
-//   pull  (bottom-up) lane per segment of a vertex v with unvisited bits U:
-//                     OR of F_L over its neighbours with early exit once U
-//                     is covered; acc[v] |= found & U
-//   fin               per vertex: new = acc & ~vis_L, vis_{L+1} = vis_L | new,
-//                     F_{L+1} = new, acc = 0, plus per-block statistics
-// Levels never leave the masks during the traversal; one final pass reads
-// each vertex's history and writes row s of lv in full cache lines: the
-// level of (s, v) is the first slot whose bit s is set (-1 if none).
-// Sources still active when the history window is full (very long
-// diameters) are finished by the single-source engine.
+// State (rank order, n vertices, H history slots):
+//   F      H x uint64[n]  frontier masks: bit s of slot L at r <=> level_s(r) = L
+//   vis    uint64[n]      visited masks (bit s <=> some level <= the current one)
+//   acc    uint64[n]      top-down accumulator (atomic OR), zero between levels
+//   rec    8 x uint64[n]  per original vertex: reached mask and 6 bit planes of
+//                         its level per source (written at the end)
+//   lv     int32[64 x n]  row s = levels from source s, original vertex order
+//
+// Level L (expands F_L into F_{L+1}), direction decided on the device from
+// the level's frontier edges against the edges of vertices still missing a
+// source (the alpha rule of the single-source heuristic, or a forced mode).
+// The unit of edge work is a 32-edge segment of a row (rows are contiguous in
+// rank order, so segment i ends where segment i+1 starts): no lane walks more
+// than 32 edges whatever the degree skew.  Chunks of XB_B64_CHUNK segments
+// are dealt to the blocks round robin, and a wave keeps, by 64-bit ballots,
+// only the segments of its chunk with work:
+//   pull (bottom-up)  segments of a vertex r with unvisited bits U = all &
+//                     ~vis[r]: OR of F_L over the segment's neighbours,
+//                     stopping once U is covered; acc[r] |= found & U
+//   push (top-down)   segments of a frontier vertex r: acc[q] |= F_L[r] for
+//                     each neighbour q
+// then, after a grid barrier, per vertex F_{L+1} = acc & ~vis, vis |=
+// F_{L+1}, acc = 0, with the level's statistics.
+// Sources still active when the H slots are used up (very long diameters)
+// are finished by the single-source engine.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "xbfs_device.h"
 
-#define XB_B64_SEG 32      /* edges per segment                       */
-#define XB_B64_BLOCK 256   /* block size of every batch kernel         */
+#define XB_B64_BLOCK 1024  /* threads per block of the batch kernel            */
 #define XB_B64_HMAX 64     /* history slots at most (levels 0..63 in one window) */
-#define XB_B64_PLANES 6    /* bit planes of a slot index < 64                  */
+#define XB_B64_PLANES 6    /* bit planes of a level < 64                         */
+#define XB_B64_OUT (8 + 2 * (XB_B64_HMAX + 2)) /* entries of the result array  */
+#define XB_B64_SEG 32      /* edges per segment                                  */
+#define XB_B64_ACC_HOT 32768 /* ranks with an accumulator line (128 B) each     */
+#ifndef XB_B64_CHUNK
+#define XB_B64_CHUNK 256   /* segments per chunk (4 per lane)                    */
+#endif
+#define XB_B64_PULL_K 16   /* neighbours gathered per round                      */
 
-// Statistic fields of the per-block partials (column = block id).
-#define XB_B64_F_NF 0    /* vertices with F_{L+1} != 0           */
-#define XB_B64_F_MF 1    /* their degree sum                     */
-#define XB_B64_F_MU 2    /* degree sum of vertices with unvisited bits */
-#define XB_B64_F_OR 3    /* OR of F_{L+1} (active sources)        */
-#define XB_B64_F_VC 4    /* final pass: reached (source, vertex) pairs */
-#define XB_B64_F_MC 5    /* final pass: their degree sum          */
-#define XB_B64_NF 6
+// Statistic fields of the per-block partials (column = block id): level
+// parity p -> 4p + {0: vertices with a new bit, 1: their degree sum, 2: degree
+// sum of vertices still missing a source, 3: OR of the new bits}; the final
+// pass: 8 + {0: reached (source, vertex) pairs, 1: their degree sum}.
+#define XB_B64_F_LVL(p, k) (4 * ((p) & 1) + (k))
+#define XB_B64_F_VC 8
+#define XB_B64_F_MC 9
 
 struct XbB64Args {
-    const int32_t *deg;
-    const int32_t *adj;
-    const int32_t *seg_u;
-    const int64_t *seg_e;
-    int64_t nseg;
-    const uint64_t *F_cur;
-    uint64_t *F_next;
-    uint64_t *acc;
-    const uint64_t *vis_cur;
-    uint64_t *vis_next;
-    long long *part; // XB_B64_NF x pstride
-    int64_t pstride;
-    int64_t n;
-    uint64_t all; // mask of the batch's sources
+    const int64_t *rbeg;  // int64[n+1], rank order
+    const int32_t *radj;  // int32[m], neighbours by rank, ascending per row
+    const int32_t *rdeg;  // int32[n], degree of rank r (descending)
+    const int32_t *seg_r; // int32[nseg], the rank whose row holds segment i
+    const int64_t *seg_e; // int64[nseg+1], first edge of segment i (seg_e[nseg] = m)
+    const int32_t *rank;  // original id -> rank
+    const int32_t *ord;   // rank -> original id
+    uint64_t *F;          // H slots of n
+    uint64_t *vis, *acc;
+    uint64_t *rec;        // 8 words per original vertex
+    int32_t *lv;          // nsrc rows of n, original order
+    long long *part;      // per-block partial statistics, XB_PMAX columns per field
+    XbCtl *ctl;
+    long long *out;       // [0] levels expanded, [1] last level's OR of new bits,
+                          // [2] frontier count of the last level, [3] v_cc, [4] m_cc,
+                          // [8 + L] wall clock at the start of level L (L <= levels
+                          // expanded), [8 + XB_B64_HMAX] between the two passes of
+                          // the level rows, [8 + XB_B64_HMAX + 1] after them,
+                          // [8 + XB_B64_HMAX + 2 + L] direction of level L (1 pull)
+    const int32_t *srcs;
+    const int8_t *sched;
+    int64_t n, m, n_nz, nseg;
+    uint64_t all;
+    int32_t nsrc, H, policy, sched_len;
+    float alpha;
 };
 
 // ---------------------------------------------------------------------------
-// set-up: degrees and segments (once per context)
+// set-up of the relabelled graph (once per context)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(XB_B64_BLOCK) k_b64_deg(const int64_t *__restrict__ beg, int64_t n,
-                                                          int32_t *__restrict__ deg,
-                                                          int32_t *__restrict__ nseg)
+__global__ void __launch_bounds__(256) k_b64_deg_key(const int64_t *__restrict__ beg, int64_t n,
+                                                     uint32_t *__restrict__ key,
+                                                     int32_t *__restrict__ id)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
-        const int64_t d = beg[v + 1] - beg[v];
-        deg[v] = (int32_t)d;
-        nseg[v] = (int32_t)((d + XB_B64_SEG - 1) / XB_B64_SEG);
+        key[v] = (uint32_t)(beg[v + 1] - beg[v]);
+        id[v] = (int32_t)v;
     }
 }
 
-// off = exclusive scan of the per-vertex segment counts
-__global__ void __launch_bounds__(XB_B64_BLOCK) k_b64_seg_fill(const int64_t *__restrict__ beg, int64_t n,
-                                                               const int32_t *__restrict__ cnt,
-                                                               const long long *__restrict__ off,
-                                                               int32_t *__restrict__ seg_u,
-                                                               int64_t *__restrict__ seg_e,
-                                                               int64_t nseg, int64_t m)
+// rank[ord[r]] = r, rdeg[r] = degree of ord[r]
+__global__ void __launch_bounds__(256) k_b64_rank(const int32_t *__restrict__ ord,
+                                                  const uint32_t *__restrict__ key_sorted, int64_t n,
+                                                  int32_t *__restrict__ rank, int32_t *__restrict__ rdeg)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+        rank[ord[r]] = (int32_t)r;
+        rdeg[r] = (int32_t)key_sorted[r];
+    }
+}
+
+__global__ void k_b64_set_end(int64_t *rbeg, int64_t n, int64_t m)
+{
+    if (threadIdx.x == 0) rbeg[n] = m;
+}
+
+// segments per rank: ceil(degree / XB_B64_SEG)
+__global__ void __launch_bounds__(256) k_b64_seg_count(const int32_t *__restrict__ rdeg, int64_t n,
+                                                       int32_t *__restrict__ cnt)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride)
+        cnt[r] = (rdeg[r] + XB_B64_SEG - 1) / XB_B64_SEG;
+}
+
+// off = exclusive scan of the segment counts
+__global__ void __launch_bounds__(256) k_b64_seg_fill(const int64_t *__restrict__ rbeg, int64_t n,
+                                                      const int32_t *__restrict__ cnt,
+                                                      const long long *__restrict__ off,
+                                                      int32_t *__restrict__ seg_r,
+                                                      int64_t *__restrict__ seg_e, int64_t nseg,
+                                                      int64_t m)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (int64_t v = gtid; v < n; v += stride) {
-        const int32_t c = cnt[v];
-        const int64_t o = off[v], b = beg[v];
+    for (int64_t r = gtid; r < n; r += stride) {
+        const int32_t c = cnt[r];
+        const int64_t o = off[r], b = rbeg[r];
         for (int32_t k = 0; k < c; k++) {
-            seg_u[o + k] = (int32_t)v;
+            seg_r[o + k] = (int32_t)r;
             seg_e[o + k] = b + (int64_t)k * XB_B64_SEG;
         }
     }
     if (gtid == 0) seg_e[nseg] = m;
 }
 
+// Row r of the relabelled graph: row ord[r] of the input with every id
+// replaced by its rank (one wave per row; the per-row sort follows).
+__global__ void __launch_bounds__(256) k_b64_rows(const int64_t *__restrict__ beg,
+                                                  const int32_t *__restrict__ adj,
+                                                  const int32_t *__restrict__ ord,
+                                                  const int32_t *__restrict__ rank,
+                                                  const int64_t *__restrict__ rbeg, int64_t n,
+                                                  int32_t *__restrict__ radj)
+{
+    const int64_t nwv = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int ln = xb_lane();
+    for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < n; r += nwv) {
+        const int32_t v = ord[r];
+        const int64_t b = beg[v], d = beg[v + 1] - b, o = rbeg[r];
+        for (int64_t j = ln; j < d; j += XB_WAVE) radj[o + j] = rank[adj[b + j]];
+    }
+}
+
 // ---------------------------------------------------------------------------
-// per batch
+// the batch kernel
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(XB_B64_BLOCK) k_b64_clear(uint64_t *__restrict__ vis0,
-                                                            uint64_t *__restrict__ F0,
-                                                            uint64_t *__restrict__ acc, int64_t n)
+// Accumulator word of rank r.  The top-down atomics of a level concentrate on
+// the hubs, which rank order packs into a few cache lines; device-scope
+// atomics on one line serialise, so each of the XB_B64_ACC_HOT first ranks
+// gets a 128-byte line of its own.  The array holds n + 15 * XB_B64_ACC_HOT
+// words.
+__device__ __forceinline__ int64_t xb_b64_acc(int64_t r)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
-        vis0[v] = 0ull;
-        F0[v] = 0ull;
-        acc[v] = 0ull;
-    }
+    return r < XB_B64_ACC_HOT ? r * 16 : r + 15 * (int64_t)XB_B64_ACC_HOT;
 }
 
-// One block of 64 threads: bit i of vis_0 and F_0 at srcs[i]; level-0
-// statistics (distinct seed vertices, their degree sum) into out[0..1].
-__global__ void __launch_bounds__(64) k_b64_seed(const int32_t *__restrict__ srcs, int32_t nsrc,
-                                                 const int32_t *__restrict__ deg,
-                                                 uint64_t *__restrict__ vis0,
-                                                 uint64_t *__restrict__ F0,
-                                                 long long *__restrict__ out)
+__device__ __forceinline__ unsigned long long xb_wave_or_u64(unsigned long long x)
 {
-    const int i = threadIdx.x;
-    unsigned long long nf = 0, mf = 0;
-    if (i < nsrc) {
-        const int32_t s = srcs[i];
-        const unsigned long long bit = 1ull << i;
-        atomicOr(reinterpret_cast<unsigned long long *>(&vis0[s]), bit);
-        atomicOr(reinterpret_cast<unsigned long long *>(&F0[s]), bit);
-        bool first = true;
-        for (int j = 0; j < i; j++)
-            if (srcs[j] == s) first = false;
-        if (first && deg[s] > 0) {
-            nf = 1;
-            mf = (unsigned long long)deg[s];
-        }
-    }
-    nf = xb_wave_sum_u64(nf);
-    mf = xb_wave_sum_u64(mf);
-    if (i == 0) {
-        out[0] = (long long)nf;
-        out[1] = (long long)mf;
-    }
+    for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o);
+    return x;
 }
 
-// Top-down: lane per segment of a frontier vertex.
-__global__ void __launch_bounds__(XB_B64_BLOCK) k_b64_push(XbB64Args a)
+// Block totals of 4 level statistics (sums, and an OR for k = 3) into column
+// bid of their fields (thread 0, plain stores: only this block writes the
+// column).  Every thread of the block calls it.
+__device__ void xb_b64_store4(const XbB64Args &a, int bid, int p, unsigned long long (&v)[4])
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.nseg; i += stride) {
-        const uint64_t f = a.F_cur[a.seg_u[i]];
-        if (!f) continue;
-        const int64_t e1 = a.seg_e[i + 1];
-        for (int64_t e = a.seg_e[i]; e < e1; e++) {
-            const int32_t v = a.adj[e];
-            const uint64_t m = f & ~a.vis_cur[v];
-            if (m) atomicOr(reinterpret_cast<unsigned long long *>(&a.acc[v]), (unsigned long long)m);
-        }
-    }
-}
-
-// Bottom-up: lane per segment of a vertex with unvisited bits; neighbours in
-// rounds of XB_B64_PULL_K (all adjacency words of a round loaded, then all
-// their frontier masks gathered, each group independent, so a round costs two
-// dependent memory trips), early exit once the unvisited bits are all
-// covered.  On the level where most vertices still miss some source, nearly
-// every segment is scanned whole, and the number of trips per segment bounds
-// the level.
-#ifndef XB_B64_PULL_K
-#define XB_B64_PULL_K 16
-#endif
-__global__ void __launch_bounds__(XB_B64_BLOCK) k_b64_pull(XbB64Args a)
-{
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.nseg; i += stride) {
-        const int32_t v = a.seg_u[i];
-        const uint64_t U = a.all & ~a.vis_cur[v];
-        if (!U) continue;
-        const int64_t e1 = a.seg_e[i + 1];
-        uint64_t got = 0;
-        for (int64_t e = a.seg_e[i]; e < e1; e += XB_B64_PULL_K) {
-            int32_t u[XB_B64_PULL_K];
-#pragma unroll
-            for (int k = 0; k < XB_B64_PULL_K; k++) u[k] = e + k < e1 ? a.adj[e + k] : -1;
-            uint64_t f[XB_B64_PULL_K];
-#pragma unroll
-            for (int k = 0; k < XB_B64_PULL_K; k++) f[k] = u[k] >= 0 ? a.F_cur[u[k]] : 0ull;
-#pragma unroll
-            for (int k = 0; k < XB_B64_PULL_K; k++) got |= f[k];
-            if ((got & U) == U) break;
-        }
-        const uint64_t m = got & U;
-        if (m) atomicOr(reinterpret_cast<unsigned long long *>(&a.acc[v]), (unsigned long long)m);
-    }
-}
-
-// Block totals of K fields into column bid of the partials (thread 0, plain
-// store: only this block writes the column).  op_or[k]: OR instead of sum.
-template <int K>
-__device__ void xb_b64_part_store(long long *part, int64_t pstride, int bid,
-                                  unsigned long long (&v)[K], const bool (&op_or)[K])
-{
-    __shared__ unsigned long long sh[K][XB_B64_BLOCK / 64];
+    __shared__ unsigned long long sh[4][XB_B64_BLOCK / 64];
     const int w = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        unsigned long long x = v[k];
-        for (int o = 32; o > 0; o >>= 1) {
-            const unsigned long long y = __shfl_xor(x, o);
-            x = op_or[k] ? (x | y) : (x + y);
-        }
-        if (xb_lane() == 0) sh[k][w] = x;
+    for (int k = 0; k < 3; k++) {
+        const unsigned long long t = xb_wave_sum_u64(v[k]);
+        if (xb_lane() == 0) sh[k][w] = t;
+    }
+    {
+        const unsigned long long t = xb_wave_or_u64(v[3]);
+        if (xb_lane() == 0) sh[3][w] = t;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        const int nw = (int)(blockDim.x >> 6);
+        unsigned long long t[4] = {0, 0, 0, 0};
+        for (int i = 0; i < nw; i++) {
+            t[0] += sh[0][i];
+            t[1] += sh[1][i];
+            t[2] += sh[2][i];
+            t[3] |= sh[3][i];
+        }
+        for (int k = 0; k < 4; k++) a.part[(int64_t)XB_B64_F_LVL(p, k) * XB_PMAX + bid] = (long long)t[k];
+    }
+    __syncthreads();
+}
+
+// Level statistics of parity p summed over nb columns by wave 0 (all loads
+// in flight at once); the results land in sh[0..3] for the whole block.
+__device__ void xb_b64_sum4(const XbB64Args &a, int nb, int p, unsigned long long *sh)
+{
+    if (threadIdx.x < 64) {
+        const int ln = xb_lane();
+        unsigned long long s0 = 0, s1 = 0, s2 = 0, o3 = 0;
+        for (int b0 = 0; b0 < nb; b0 += 256) {
+            unsigned long long v[4][4];
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            unsigned long long t = 0;
-            for (int j = 0; j < XB_B64_BLOCK / 64; j++) t = op_or[k] ? (t | sh[k][j]) : (t + sh[k][j]);
-            part[(int64_t)k * pstride + bid] = (long long)t;
-        }
-    }
-}
-
-__global__ void __launch_bounds__(XB_B64_BLOCK) k_b64_fin(XbB64Args a)
-{
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    unsigned long long nf = 0, mf = 0, mu = 0, orr = 0;
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += stride) {
-        const uint64_t nb = a.acc[v];
-        const uint64_t vp = a.vis_cur[v];
-        const uint64_t nw = nb & ~vp;
-        const uint64_t vn = vp | nw;
-        a.vis_next[v] = vn;
-        a.F_next[v] = nw;
-        if (nb) a.acc[v] = 0ull;
-        const bool has_u = (a.all & ~vn) != 0ull;
-        if (nw || has_u) {
-            const unsigned long long d = (unsigned long long)a.deg[v];
-            if (nw) {
-                nf += 1;
-                mf += d;
-                orr |= nw;
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int b = b0 + ln + 64 * j;
+                    v[k][j] = b < nb ? (unsigned long long)a.part[(int64_t)XB_B64_F_LVL(p, k) * XB_PMAX + b]
+                                     : 0ull;
+                }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                s0 += v[0][j];
+                s1 += v[1][j];
+                s2 += v[2][j];
+                o3 |= v[3][j];
             }
-            if (has_u) mu += d;
+        }
+        s0 = xb_wave_sum_u64(s0);
+        s1 = xb_wave_sum_u64(s1);
+        s2 = xb_wave_sum_u64(s2);
+        o3 = xb_wave_or_u64(o3);
+        if (ln == 0) {
+            sh[0] = s0;
+            sh[1] = s1;
+            sh[2] = s2;
+            sh[3] = o3;
         }
     }
-    unsigned long long vv[4] = {nf, mf, mu, orr};
-    const bool op[4] = {false, false, false, true};
-    xb_b64_part_store<4>(a.part, a.pstride, blockIdx.x, vv, op);
+    __syncthreads();
 }
 
-// Sums (or ORs) fields [f0, f0+k) over nb columns into out[0..k).
-__global__ void __launch_bounds__(1024) k_b64_reduce(const long long *__restrict__ part, int64_t pstride,
-                                                     int32_t nb, int32_t f0, int32_t k, int32_t or_field,
-                                                     long long *__restrict__ out)
+// Direction of level L: 1 pull, 0 push.
+__device__ __forceinline__ int xb_b64_mode(const XbB64Args &a, int32_t L, unsigned long long m_f,
+                                           unsigned long long m_u)
 {
-    __shared__ unsigned long long sh[16];
-    for (int f = f0; f < f0 + k; f++) {
-        const bool use_or = f == or_field;
-        unsigned long long x = 0;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-            const unsigned long long y = (unsigned long long)part[(int64_t)f * pstride + b];
-            x = use_or ? (x | y) : (x + y);
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            const unsigned long long y = __shfl_xor(x, o);
-            x = use_or ? (x | y) : (x + y);
-        }
-        if (xb_lane() == 0) sh[threadIdx.x >> 6] = x;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long t = 0;
-            for (int j = 0; j < (int)(blockDim.x >> 6); j++) t = use_or ? (t | sh[j]) : (t + sh[j]);
-            out[f - f0] = (long long)t;
-        }
-        __syncthreads();
+    switch (a.policy) {
+    case XBFS_POLICY_FORCE_TD_SF:
+    case XBFS_POLICY_FORCE_TD_SS: return 0;
+    case XBFS_POLICY_FORCE_BU: return 1;
+    case XBFS_POLICY_SCHEDULE:
+        if (L < a.sched_len) return a.sched[L] == XBFS_MODE_BU ? 1 : 0;
+        break;
+    default: break;
     }
+    return (double)m_f > (double)a.alpha * (double)m_u ? 1 : 0;
 }
 
-// Final pass: row s of lv for every source s in `done`, lane per vertex
-// (a wave covers 64 consecutive vertices, so every row store is one
-// coalesced 256-byte run).  The history is monotone, so bit s first appears
-// in exactly one slot j0 = level_s(v); the pass reads the slots once and
-// builds bit planes P_k (bit s of P_k = bit k of j0) from the newly set bits
-// of each slot, then every level is a few bit extracts.  Unreached: -1.
-// Also counts reached (source, vertex) pairs and their degree sum per block.
-__global__ void __launch_bounds__(XB_B64_BLOCK) k_b64_flush(const uint64_t *__restrict__ hist,
-                                                            int32_t nslots, int64_t n, int32_t nsrc,
-                                                            uint64_t done, const int32_t *__restrict__ deg,
-                                                            int32_t *__restrict__ lv,
-                                                            long long *__restrict__ part, int64_t pstride)
+// Builds the wave's list of the chunk's segments with work: bit set in
+// `want` per lane and round k (segment base + 64 k + lane); ascending order.
+__device__ __forceinline__ int xb_b64_list(int32_t *list, int64_t base, unsigned want)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int ln = xb_lane();
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < XB_B64_CHUNK / 64; k++) {
+        const bool w = (want >> k) & 1u;
+        const unsigned long long mk = __ballot(w);
+        if (w) list[cnt + xb_rank(mk)] = (int32_t)(base + 64 * k + ln);
+        cnt += (int)__popcll(mk);
+    }
+    xb_wave_sync();
+    return cnt;
+}
+
+__global__ void __launch_bounds__(XB_B64_BLOCK) k_b64_persistent(XbB64Args a)
+{
+    __shared__ int32_t s_list[XB_B64_BLOCK / 64][XB_B64_CHUNK];
+    __shared__ unsigned long long s_st[4];
+    const int bid = blockIdx.x, nb = gridDim.x, ln = xb_lane();
+    const int64_t gtid = (int64_t)bid * blockDim.x + threadIdx.x;
+    const int64_t nthr = (int64_t)nb * blockDim.x;
+    const int64_t nwv = nthr >> 6, wrr = (int64_t)(threadIdx.x >> 6) * nb + bid;
+    int32_t *list = s_list[threadIdx.x >> 6];
+    const int64_t n = a.n, n_nz = a.n_nz;
+    const int64_t nchunks = (a.nseg + XB_B64_CHUNK - 1) / XB_B64_CHUNK;
+
+    // clear: visited, accumulator, slot 0
+    for (int64_t v = gtid; v < n; v += nthr) {
+        a.vis[v] = 0ull;
+        a.acc[xb_b64_acc(v)] = 0ull;
+        a.F[v] = 0ull;
+    }
+    if (!xb_grid_sync(a.ctl)) return;
+    // seed: bit i at srcs[i]; level-0 statistics (distinct seeds with edges)
+    // in block 0's column, zeros in the others
+    {
+        unsigned long long v4[4] = {0, 0, 0, 0};
+        if (bid == 0 && threadIdx.x < 64) {
+            const int i = threadIdx.x;
+            if (i < a.nsrc) {
+                const int32_t r = a.rank[a.srcs[i]];
+                const unsigned long long bit = 1ull << i;
+                atomicOr(reinterpret_cast<unsigned long long *>(&a.F[r]), bit);
+                atomicOr(reinterpret_cast<unsigned long long *>(&a.vis[r]), bit);
+                bool first = true;
+                for (int j = 0; j < i; j++)
+                    if (a.srcs[j] == a.srcs[i]) first = false;
+                if (first && a.rdeg[r] > 0) {
+                    v4[0] = 1;
+                    v4[1] = (unsigned long long)a.rdeg[r];
+                }
+                v4[3] = bit;
+            }
+            if (i == 0) v4[2] = (unsigned long long)a.m;
+        }
+        xb_b64_store4(a, bid, 0, v4);
+    }
+    if (!xb_grid_sync(a.ctl)) return;
+
+    int32_t L = 0;
+    unsigned long long last_or = 0, last_nf = 0;
+    for (;;) {
+        if (bid == 0 && threadIdx.x == 0) a.out[8 + L] = (long long)wall_clock64();
+        xb_b64_sum4(a, nb, L, s_st);
+        const unsigned long long n_f = s_st[0], m_f = s_st[1], m_u = s_st[2], orr = s_st[3];
+        last_or = orr;
+        last_nf = n_f;
+        if (n_f == 0 || L + 1 >= a.H) break;
+        const int pull = xb_b64_mode(a, L, m_f, m_u);
+        if (bid == 0 && threadIdx.x == 0) a.out[8 + XB_B64_HMAX + 2 + L] = pull;
+        const uint64_t *Fc = a.F + (int64_t)L * n;
+        uint64_t *Fn = a.F + (int64_t)(L + 1) * n;
+        // segments with work, chunk by chunk; chunks are dealt to the blocks
+        // round robin (chunk c to block c mod nb), so the few chunks that hold
+        // a level's work (the hub rows of a top-down level) spread over every
+        // CU instead of filling the first few
+        for (int64_t ch = wrr; ch < nchunks; ch += nwv) {
+            const int64_t base = ch * XB_B64_CHUNK;
+            int32_t sr[XB_B64_CHUNK / 64];
+#pragma unroll
+            for (int k = 0; k < XB_B64_CHUNK / 64; k++) {
+                const int64_t i = base + 64 * k + ln;
+                sr[k] = i < a.nseg ? a.seg_r[i] : -1;
+            }
+            unsigned want = 0;
+            if (pull) {
+                uint64_t vp[XB_B64_CHUNK / 64];
+#pragma unroll
+                for (int k = 0; k < XB_B64_CHUNK / 64; k++) vp[k] = sr[k] >= 0 ? a.vis[sr[k]] : a.all;
+#pragma unroll
+                for (int k = 0; k < XB_B64_CHUNK / 64; k++)
+                    if ((a.all & ~vp[k]) != 0ull) want |= 1u << k;
+            } else {
+                uint64_t f[XB_B64_CHUNK / 64];
+#pragma unroll
+                for (int k = 0; k < XB_B64_CHUNK / 64; k++) f[k] = sr[k] >= 0 ? Fc[sr[k]] : 0ull;
+#pragma unroll
+                for (int k = 0; k < XB_B64_CHUNK / 64; k++)
+                    if (f[k] != 0ull) want |= 1u << k;
+            }
+            const int cnt = xb_b64_list(list, base, want);
+            for (int i = ln; i < cnt; i += 64) {
+                const int32_t sg = list[i];
+                const int32_t r = a.seg_r[sg];
+                const int64_t e0 = a.seg_e[sg], e1 = a.seg_e[sg + 1];
+                if (pull) {
+                    // bits another segment of the row already found are left out
+                    const uint64_t U = a.all & ~a.vis[r] & ~a.acc[xb_b64_acc(r)];
+                    if (!U) continue;
+                    uint64_t got = 0;
+                    for (int64_t e = e0; e < e1; e += XB_B64_PULL_K) {
+                        int32_t u[XB_B64_PULL_K];
+#pragma unroll
+                        for (int k = 0; k < XB_B64_PULL_K; k++) u[k] = e + k < e1 ? a.radj[e + k] : -1;
+                        uint64_t f[XB_B64_PULL_K];
+#pragma unroll
+                        for (int k = 0; k < XB_B64_PULL_K; k++) f[k] = u[k] >= 0 ? Fc[u[k]] : 0ull;
+#pragma unroll
+                        for (int k = 0; k < XB_B64_PULL_K; k++) got |= f[k];
+                        if ((got & U) == U) break;
+                    }
+                    const uint64_t nw = got & U;
+                    if (nw)
+                        atomicOr(reinterpret_cast<unsigned long long *>(&a.acc[xb_b64_acc(r)]),
+                                 (unsigned long long)nw);
+                } else {
+                    const uint64_t f = Fc[r];
+                    for (int64_t e = e0; e < e1; e += 8) {
+                        int32_t u[8];
+#pragma unroll
+                        for (int k = 0; k < 8; k++) u[k] = e + k < e1 ? a.radj[e + k] : -1;
+                        // an atomic only where the accumulator misses some of
+                        // the bits (hub targets receive the same bits from many
+                        // frontier rows)
+                        uint64_t have[8];
+#pragma unroll
+                        for (int k = 0; k < 8; k++) have[k] = u[k] >= 0 ? a.acc[xb_b64_acc(u[k])] : f;
+#pragma unroll
+                        for (int k = 0; k < 8; k++)
+                            if ((have[k] & f) != f)
+                                atomicOr(reinterpret_cast<unsigned long long *>(&a.acc[xb_b64_acc(u[k])]),
+                                         (unsigned long long)f);
+                    }
+                }
+            }
+            xb_wave_sync();
+        }
+        if (!xb_grid_sync(a.ctl)) return;
+        // the accumulated bits become level L+1 where they are new
+        unsigned long long st[4] = {0, 0, 0, 0}; // new vertices, their degrees, still-missing degrees, OR
+        for (int64_t r = gtid; r < n_nz; r += nthr) {
+            const int64_t ai = xb_b64_acc(r);
+            const uint64_t ac = a.acc[ai];
+            const uint64_t vp = a.vis[r];
+            const uint64_t nw = ac & ~vp;
+            Fn[r] = nw;
+            if (ac) a.acc[ai] = 0ull;
+            const unsigned long long d = (unsigned long long)a.rdeg[r];
+            if (nw) {
+                a.vis[r] = vp | nw;
+                st[0] += 1;
+                st[1] += d;
+                st[3] |= nw;
+            }
+            if (a.all & ~(vp | nw)) st[2] += d;
+        }
+        xb_b64_store4(a, bid, L + 1, st);
+        if (!xb_grid_sync(a.ctl)) return;
+        L++;
+    }
+
+    // Level rows.  Pass 1 (rank order, coalesced slot reads): each vertex's
+    // reached mask and the 6 bit planes of its levels (bit s of plane k = bit
+    // k of level_s), one 64-byte record at its original id; a source still
+    // active at the end of the window (finished by the single-source engine)
+    // is left out.  Pass 2 (original order): every row s gets the vertex's
+    // level in one coalesced 256-byte run per wave.
+    const uint64_t unfinished = last_nf ? (last_or & a.all) : 0ull;
+    const uint64_t done = a.all & ~unfinished;
+    const int32_t nslots = L + 1;
     unsigned long long vc = 0, mc = 0;
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
-        uint64_t prev = 0, P[XB_B64_PLANES];
+    for (int64_t r = gtid; r < n; r += nthr) {
+        const int32_t ns = r < n_nz ? nslots : 1; // isolated: only a seed has bits
+        uint64_t reached = 0, P[XB_B64_PLANES];
 #pragma unroll
         for (int k = 0; k < XB_B64_PLANES; k++) P[k] = 0ull;
-        for (int j = 0; j < nslots; j++) {
-            const uint64_t h = hist[(int64_t)j * n + v];
-            const uint64_t nw = h & ~prev;
-            prev = h;
+        for (int32_t j = 0; j < ns; j++) {
+            const uint64_t f = a.F[(int64_t)j * n + r];
+            reached |= f;
 #pragma unroll
             for (int k = 0; k < XB_B64_PLANES; k++)
-                if ((j >> k) & 1) P[k] |= nw;
+                if ((j >> k) & 1) P[k] |= f;
         }
-        const uint64_t reached = prev & done;
+        reached &= done;
         vc += (unsigned long long)__popcll(reached);
-        mc += (unsigned long long)__popcll(reached) * (unsigned long long)deg[v];
-        // 32-bit halves: bit extracts on 32-bit words
-        for (int half = 0; half < 2; half++) {
-            const uint32_t r = (uint32_t)(reached >> (32 * half));
-            const uint32_t dn = (uint32_t)(done >> (32 * half));
-            uint32_t q[XB_B64_PLANES];
+        mc += (unsigned long long)__popcll(reached) * (unsigned long long)a.rdeg[r];
+        ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(a.rec + (int64_t)a.ord[r] * 8);
+        dst[0] = make_ulonglong2(reached, P[0]);
+        dst[1] = make_ulonglong2(P[1], P[2]);
+        dst[2] = make_ulonglong2(P[3], P[4]);
+        dst[3] = make_ulonglong2(P[5], 0ull);
+    }
+    if (!xb_grid_sync(a.ctl)) return;
+    if (bid == 0 && threadIdx.x == 0) a.out[8 + XB_B64_HMAX] = (long long)wall_clock64();
+    for (int64_t v = gtid; v < n; v += nthr) {
+        const ulonglong2 *srcp = reinterpret_cast<const ulonglong2 *>(a.rec + v * 8);
+        const ulonglong2 q0 = srcp[0], q1 = srcp[1], q2 = srcp[2], q3 = srcp[3];
+        const uint64_t reached = q0.x;
+        const uint64_t P[XB_B64_PLANES] = {q0.y, q1.x, q1.y, q2.x, q2.y, q3.x};
+        for (int s = 0; s < a.nsrc; s++) {
+            if (!((done >> s) & 1ull)) continue;
+            int32_t lvl = 0;
 #pragma unroll
-            for (int k = 0; k < XB_B64_PLANES; k++) q[k] = (uint32_t)(P[k] >> (32 * half));
-            const int s0 = 32 * half;
-            const int s1 = nsrc < s0 + 32 ? nsrc : s0 + 32;
-            for (int s = s0; s < s1; s++) {
-                const int b = s - s0;
-                if (!((dn >> b) & 1u)) continue;
-                int32_t lvl = 0;
-#pragma unroll
-                for (int k = 0; k < XB_B64_PLANES; k++) lvl |= (int32_t)((q[k] >> b) & 1u) << k;
-                lv[(int64_t)s * n + v] = ((r >> b) & 1u) ? lvl : -1;
-            }
+            for (int k = 0; k < XB_B64_PLANES; k++) lvl |= (int32_t)((P[k] >> s) & 1ull) << k;
+            __builtin_nontemporal_store(((reached >> s) & 1ull) ? lvl : -1, &a.lv[(int64_t)s * n + v]);
         }
     }
-    unsigned long long vv[2] = {vc, mc};
-    const bool op[2] = {false, false};
-    long long *p = part + (int64_t)XB_B64_F_VC * pstride;
-    xb_b64_part_store<2>(p, pstride, blockIdx.x, vv, op);
+    {
+        __shared__ unsigned long long sh[2][XB_B64_BLOCK / 64];
+        const int w = threadIdx.x >> 6;
+        const unsigned long long t0 = xb_wave_sum_u64(vc), t1 = xb_wave_sum_u64(mc);
+        if (ln == 0) {
+            sh[0][w] = t0;
+            sh[1][w] = t1;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long x0 = 0, x1 = 0;
+            for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+                x0 += sh[0][i];
+                x1 += sh[1][i];
+            }
+            a.part[(int64_t)XB_B64_F_VC * XB_PMAX + bid] = (long long)x0;
+            a.part[(int64_t)XB_B64_F_MC * XB_PMAX + bid] = (long long)x1;
+        }
+    }
+    if (!xb_grid_sync(a.ctl)) return;
+    if (bid == 0 && threadIdx.x < 64) {
+        unsigned long long x0 = 0, x1 = 0;
+        for (int b = ln; b < nb; b += 64) {
+            x0 += (unsigned long long)a.part[(int64_t)XB_B64_F_VC * XB_PMAX + b];
+            x1 += (unsigned long long)a.part[(int64_t)XB_B64_F_MC * XB_PMAX + b];
+        }
+        x0 = xb_wave_sum_u64(x0);
+        x1 = xb_wave_sum_u64(x1);
+        if (ln == 0) {
+            a.out[8 + XB_B64_HMAX + 1] = (long long)wall_clock64();
+            a.out[0] = L;
+            a.out[1] = (long long)unfinished;
+            a.out[2] = (long long)last_nf;
+            a.out[3] = (long long)x0;
+            a.out[4] = (long long)x1;
+        }
+    }
 }

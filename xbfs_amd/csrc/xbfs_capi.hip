@@ -5,11 +5,13 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <climits>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_segmented_radix_sort.hpp>
 
 #include "xbfs.h"
@@ -56,8 +58,10 @@ int xb_ctx_free_device(xbfs_ctx *c)
                     c->d_recs,  c->d_sched, c->d_mlp,
                     c->d_list,  c->d_ubits, c->d_tile, c->d_list_cnt, c->s_level,
                     c->s_vis,   c->s_fbm,   c->s_qbuf, c->s_ctl, c->d_part, c->s_part,
-                    c->d_srcs,  c->d_b64_deg, c->d_b64_segu, c->d_b64_sege, c->d_b64_F,
-                    c->d_b64_acc, c->d_b64_hist, c->d_b64_lv, c->d_b64_out, c->d_b64_srcs};
+                    c->d_srcs,  c->d_b64_rank, c->d_b64_ord, c->d_b64_rdeg, c->d_b64_rbeg,
+                    c->d_b64_radj, c->d_b64_segr, c->d_b64_sege, c->d_b64_F, c->d_b64_vis,
+                    c->d_b64_acc, c->d_b64_rec,
+                    c->d_b64_lv, c->d_b64_out, c->d_b64_srcs};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
@@ -766,85 +770,151 @@ int xbfs_autotune(xbfs_ctx *c, const int32_t *srcs, int32_t nsrc, int32_t multi,
 // ---------------------------------------------------------------------------
 // 64-source batch (kernels in xbfs_batch.hip)
 // ---------------------------------------------------------------------------
-#define XB_B64_HIST_BUDGET (16ll << 30) /* bytes of visited history at most */
+#define XB_B64_HIST_BUDGET (16ll << 30) /* bytes of frontier history at most */
 
-// Degrees, 32-edge segments and the batch buffers, once per context.
+// The relabelled graph (vertices by degree, descending; rows by neighbour
+// rank, ascending) and the batch buffers, once per context.
 static int b64_setup(xbfs_ctx *c)
 {
     if (c->b64_ready) return XBFS_OK;
     const int64_t n = c->n, m = c->m;
-    const int grid = c->num_cu * 8;
-    int32_t *cnt = nullptr;
-    long long *off = nullptr, *part = nullptr;
+    if (n > INT32_MAX) return XBFS_ERR_UNSUPPORTED;
+    const int grid = grid_for(n, 256, 65536);
+    uint32_t *kin = nullptr, *kout = nullptr;
+    int32_t *ids = nullptr, *radj_in = nullptr;
+    long long *spart = nullptr, *soff = nullptr;
+    void *tmp = nullptr;
     const int64_t T = ceil_div(n, XB_SCAN_CHUNK);
+    int end_bit = 1;
+    while (end_bit < 32 && (n >> end_bit) != 0) end_bit++;
     int rc = XBFS_OK;
-    if (hipMalloc(&c->d_b64_deg, sizeof(int32_t) * (size_t)n) != hipSuccess ||
-        hipMalloc(&cnt, sizeof(int32_t) * (size_t)n) != hipSuccess ||
-        hipMalloc(&off, sizeof(long long) * (size_t)n) != hipSuccess ||
-        hipMalloc(&part, sizeof(long long) * (size_t)T) != hipSuccess)
-        rc = XBFS_ERR_NOMEM;
-    int64_t nseg = 0;
+    auto cleanup = [&]() {
+        for (void *p : {(void *)kin, (void *)kout, (void *)ids, (void *)radj_in, (void *)spart,
+                        (void *)soff, tmp})
+            if (p) (void)hipFree(p);
+        (void)hipGetLastError();
+    };
+    if (hipMalloc(&c->d_b64_rank, sizeof(int32_t) * (size_t)n) != hipSuccess ||
+        hipMalloc(&c->d_b64_ord, sizeof(int32_t) * (size_t)n) != hipSuccess ||
+        hipMalloc(&c->d_b64_rdeg, sizeof(int32_t) * (size_t)n) != hipSuccess ||
+        hipMalloc(&c->d_b64_rbeg, sizeof(int64_t) * (size_t)(n + 1)) != hipSuccess ||
+        hipMalloc(&c->d_b64_radj, sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1)) != hipSuccess ||
+        hipMalloc(&kin, sizeof(uint32_t) * (size_t)n) != hipSuccess ||
+        hipMalloc(&kout, sizeof(uint32_t) * (size_t)n) != hipSuccess ||
+        hipMalloc(&ids, sizeof(int32_t) * (size_t)n) != hipSuccess ||
+        hipMalloc(&spart, sizeof(long long) * (size_t)T) != hipSuccess ||
+        hipMalloc(&soff, sizeof(long long) * (size_t)n) != hipSuccess) {
+        cleanup();
+        return XBFS_ERR_NOMEM;
+    }
+    // ranks: a stable radix sort of the vertex ids by degree, descending
+    size_t tb = 0;
+    if (rocprim::radix_sort_pairs_desc(nullptr, tb, kin, kout, ids, c->d_b64_ord, (unsigned int)n, 0,
+                                       32, c->stream) != hipSuccess ||
+        hipMalloc(&tmp, tb > 0 ? tb : 1) != hipSuccess) {
+        cleanup();
+        return XBFS_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(k_b64_deg_key, dim3(grid), dim3(256), 0, c->stream, c->d_beg, n, kin, ids);
+    if (rocprim::radix_sort_pairs_desc(tmp, tb, kin, kout, ids, c->d_b64_ord, (unsigned int)n, 0, 32,
+                                       c->stream) != hipSuccess)
+        rc = XBFS_ERR_HIP;
+    (void)hipFree(tmp);
+    tmp = nullptr;
     if (rc == XBFS_OK) {
-        hipLaunchKernelGGL(k_b64_deg, dim3(grid), dim3(XB_B64_BLOCK), 0, c->stream, c->d_beg, n,
-                           c->d_b64_deg, cnt);
-        hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)T), dim3(256), 0, c->stream, cnt, n, part);
-        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(1024), 0, c->stream, part, T);
-        hipLaunchKernelGGL(k_scan_down, dim3((unsigned)T), dim3(256), 0, c->stream, cnt, n, part, off);
+        hipLaunchKernelGGL(k_b64_rank, dim3(grid), dim3(256), 0, c->stream, c->d_b64_ord, kout, n,
+                           c->d_b64_rank, c->d_b64_rdeg);
+        // row offsets: exclusive scan of the degrees by rank
+        hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)T), dim3(256), 0, c->stream, c->d_b64_rdeg, n,
+                           spart);
+        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(1024), 0, c->stream, spart, T);
+        hipLaunchKernelGGL(k_scan_down, dim3((unsigned)T), dim3(256), 0, c->stream, c->d_b64_rdeg, n,
+                           spart, reinterpret_cast<long long *>(c->d_b64_rbeg));
+        hipLaunchKernelGGL(k_b64_set_end, dim3(1), dim3(64), 0, c->stream, c->d_b64_rbeg, n, m);
+        if (hipGetLastError() != hipSuccess) rc = XBFS_ERR_HIP;
+    }
+    // 32-edge segments: count per rank (into the ids buffer, no longer
+    // needed), exclusive scan, fill
+    if (rc == XBFS_OK) {
+        int32_t *scnt = ids;
+        hipLaunchKernelGGL(k_b64_seg_count, dim3(grid), dim3(256), 0, c->stream, c->d_b64_rdeg, n, scnt);
+        hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)T), dim3(256), 0, c->stream, scnt, n, spart);
+        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(1024), 0, c->stream, spart, T);
+        hipLaunchKernelGGL(k_scan_down, dim3((unsigned)T), dim3(256), 0, c->stream, scnt, n, spart, soff);
         long long last_off = 0;
         int32_t last_cnt = 0;
         if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(&last_off, off + (n - 1), sizeof(long long), hipMemcpyDeviceToHost,
+            hipMemcpyAsync(&last_off, soff + (n - 1), sizeof(long long), hipMemcpyDeviceToHost,
                            c->stream) != hipSuccess ||
-            hipMemcpyAsync(&last_cnt, cnt + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost,
+            hipMemcpyAsync(&last_cnt, scnt + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost,
                            c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
             rc = XBFS_ERR_HIP;
-        nseg = last_off + last_cnt;
+        const int64_t nseg = last_off + last_cnt;
+        if (rc == XBFS_OK &&
+            (nseg > INT32_MAX ||
+             hipMalloc(&c->d_b64_segr, sizeof(int32_t) * (size_t)std::max<int64_t>(nseg, 1)) != hipSuccess ||
+             hipMalloc(&c->d_b64_sege, sizeof(int64_t) * (size_t)(nseg + 1)) != hipSuccess))
+            rc = nseg > INT32_MAX ? XBFS_ERR_UNSUPPORTED : XBFS_ERR_NOMEM;
+        if (rc == XBFS_OK) {
+            hipLaunchKernelGGL(k_b64_seg_fill, dim3(grid), dim3(256), 0, c->stream, c->d_b64_rbeg, n, scnt,
+                               soff, c->d_b64_segr, c->d_b64_sege, nseg, m);
+            if (hipGetLastError() != hipSuccess) rc = XBFS_ERR_HIP;
+            c->b64_nseg = nseg;
+        }
     }
-    if (rc == XBFS_OK && (hipMalloc(&c->d_b64_segu, sizeof(int32_t) * (size_t)std::max<int64_t>(nseg, 1)) != hipSuccess ||
-                          hipMalloc(&c->d_b64_sege, sizeof(int64_t) * (size_t)(nseg + 1)) != hipSuccess))
-        rc = XBFS_ERR_NOMEM;
-    if (rc == XBFS_OK) {
-        hipLaunchKernelGGL(k_b64_seg_fill, dim3(grid), dim3(XB_B64_BLOCK), 0, c->stream, c->d_beg, n,
-                           cnt, off, c->d_b64_segu, c->d_b64_sege, nseg, m);
-        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
-            rc = XBFS_ERR_HIP;
+    // rows in rank ids, then each row sorted ascending (largest degree first);
+    // without the sort's memory or past its 32-bit counts the rows keep their
+    // input order (the traversal does not depend on it)
+    if (rc == XBFS_OK && m > 0) {
+        const bool sortable = m <= INT32_MAX && hipMalloc(&radj_in, sizeof(int32_t) * (size_t)m) == hipSuccess;
+        (void)hipGetLastError();
+        int32_t *dst = sortable ? radj_in : c->d_b64_radj;
+        hipLaunchKernelGGL(k_b64_rows, dim3(grid_for(n * 64, 256, 65536)), dim3(256), 0, c->stream,
+                           c->d_beg, c->d_adj, c->d_b64_ord, c->d_b64_rank, c->d_b64_rbeg, n, dst);
+        if (hipGetLastError() != hipSuccess) rc = XBFS_ERR_HIP;
+        if (rc == XBFS_OK && sortable) {
+            bool ok = rocprim::segmented_radix_sort_keys(nullptr, tb, radj_in, c->d_b64_radj,
+                                                         (unsigned int)m, (unsigned int)n, c->d_b64_rbeg,
+                                                         c->d_b64_rbeg + 1, 0, end_bit,
+                                                         c->stream) == hipSuccess &&
+                      hipMalloc(&tmp, tb > 0 ? tb : 1) == hipSuccess &&
+                      rocprim::segmented_radix_sort_keys(tmp, tb, radj_in, c->d_b64_radj,
+                                                         (unsigned int)m, (unsigned int)n, c->d_b64_rbeg,
+                                                         c->d_b64_rbeg + 1, 0, end_bit,
+                                                         c->stream) == hipSuccess;
+            if (!ok) {
+                // keep the unsorted rows
+                (void)hipGetLastError();
+                if (hipMemcpyAsync(c->d_b64_radj, radj_in, sizeof(int32_t) * (size_t)m,
+                                   hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+                    rc = XBFS_ERR_HIP;
+            }
+        }
+        if (hipStreamSynchronize(c->stream) != hipSuccess) rc = XBFS_ERR_HIP;
     }
-    if (cnt) (void)hipFree(cnt);
-    if (off) (void)hipFree(off);
-    if (part) (void)hipFree(part);
+    cleanup();
     if (rc) return rc;
-    c->b64_nseg = nseg;
-    const int64_t H = std::max<int64_t>(8, std::min<int64_t>(XB_B64_HMAX, XB_B64_HIST_BUDGET / (8 * n)));
+    const int64_t H = std::max<int64_t>(8, std::min<int64_t>(XB_B64_HMAX, XB_B64_HIST_BUDGET / (8 * std::max<int64_t>(n, 1))));
     c->b64_H = (int32_t)H;
-    if (hipMalloc(&c->d_b64_F, sizeof(uint64_t) * (size_t)n * 2) != hipSuccess ||
-        hipMalloc(&c->d_b64_acc, sizeof(uint64_t) * (size_t)n) != hipSuccess ||
-        hipMalloc(&c->d_b64_hist, sizeof(uint64_t) * (size_t)n * (size_t)H) != hipSuccess ||
+    if (hipMalloc(&c->d_b64_F, sizeof(uint64_t) * (size_t)n * (size_t)H) != hipSuccess ||
+        hipMalloc(&c->d_b64_vis, sizeof(uint64_t) * (size_t)n) != hipSuccess ||
+        hipMalloc(&c->d_b64_acc, sizeof(uint64_t) * (size_t)(n + 15 * XB_B64_ACC_HOT)) != hipSuccess ||
+        hipMalloc(&c->d_b64_rec, sizeof(uint64_t) * 8 * (size_t)n) != hipSuccess ||
         hipMalloc(&c->d_b64_lv, sizeof(int32_t) * (size_t)n * 64) != hipSuccess ||
-        hipMalloc(&c->d_b64_out, sizeof(long long) * 8) != hipSuccess ||
+        hipMalloc(&c->d_b64_out, sizeof(long long) * XB_B64_OUT) != hipSuccess ||
         hipMalloc(&c->d_b64_srcs, sizeof(int32_t) * 64) != hipSuccess)
         return XBFS_ERR_NOMEM;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_b64_persistent, XB_B64_BLOCK, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+        return XBFS_ERR_HIP;
+    c->b64_grid = std::min(per_cu * c->num_cu, XB_PMAX);
     for (hipEvent_t &e : c->ev_b64)
         if (!e) XB_HIP(hipEventCreate(&e));
     c->b64_ready = 1;
     return XBFS_OK;
-}
-
-// Direction of batch level L: 0 push, 1 pull.
-static int b64_mode(const xbfs_opts &o, int32_t L, long long m_f, long long m_u)
-{
-    switch (o.policy) {
-    case XBFS_POLICY_FORCE_TD_SF:
-    case XBFS_POLICY_FORCE_TD_SS: return 0;
-    case XBFS_POLICY_FORCE_BU: return 1;
-    case XBFS_POLICY_SCHEDULE:
-        if (L < o.schedule_len) return o.schedule[L] == XBFS_MODE_BU ? 1 : 0;
-        break;
-    default: break;
-    }
-    // pull once the frontier's edges exceed alpha x the edges of vertices
-    // that still miss a source
-    return (double)m_f > (double)o.alpha * (double)m_u ? 1 : 0;
 }
 
 extern "C" {
@@ -862,70 +932,79 @@ int xbfs_bfs_batch64(xbfs_ctx *c, const int32_t *srcs, int32_t nsrc, const xbfs_
     rc = b64_setup(c);
     if (rc) return rc;
     const int64_t n = c->n;
-    const int grid = c->num_cu * 8;
-    const uint64_t all = nsrc == 64 ? ~0ull : ((1ull << nsrc) - 1ull);
-    long long h_out[8] = {0};
     XbB64Args a;
-    a.deg = c->d_b64_deg;
-    a.adj = c->d_adj;
-    a.seg_u = c->d_b64_segu;
+    a.rbeg = c->d_b64_rbeg;
+    a.radj = c->d_b64_radj;
+    a.rdeg = c->d_b64_rdeg;
+    a.rank = c->d_b64_rank;
+    a.ord = c->d_b64_ord;
+    a.F = c->d_b64_F;
+    a.vis = c->d_b64_vis;
+    a.acc = c->d_b64_acc;
+    a.rec = c->d_b64_rec;
+    a.lv = c->d_b64_lv;
+    a.part = c->d_part;
+    a.ctl = c->d_ctl;
+    a.out = c->d_b64_out;
+    a.srcs = c->d_b64_srcs;
+    a.sched = c->d_sched;
+    a.n = n;
+    a.m = c->m;
+    a.n_nz = c->n_nz;
+    a.seg_r = c->d_b64_segr;
     a.seg_e = c->d_b64_sege;
     a.nseg = c->b64_nseg;
-    a.acc = c->d_b64_acc;
-    a.part = c->d_part;
-    a.pstride = XB_PMAX;
-    a.n = n;
-    a.all = all;
-    XB_HIP(hipEventRecord(c->ev_b64[0], c->stream));
+    a.all = nsrc == 64 ? ~0ull : ((1ull << nsrc) - 1ull);
+    a.nsrc = nsrc;
+    a.H = c->b64_H;
+    a.policy = o.policy;
+    a.sched_len = o.policy == XBFS_POLICY_SCHEDULE ? o.schedule_len : 0;
+    a.alpha = o.alpha;
+    int grid = c->b64_grid;
+    if (o.grid_dim > 0 && o.grid_dim < grid) grid = o.grid_dim;
     XB_HIP(hipMemcpyAsync(c->d_b64_srcs, srcs, sizeof(int32_t) * (size_t)nsrc,
                           hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_b64_clear, dim3(grid), dim3(XB_B64_BLOCK), 0, c->stream, c->d_b64_hist,
-                       c->d_b64_F, c->d_b64_acc, n);
-    hipLaunchKernelGGL(k_b64_seed, dim3(1), dim3(64), 0, c->stream, c->d_b64_srcs, nsrc,
-                       c->d_b64_deg, c->d_b64_hist, c->d_b64_F, c->d_b64_out);
+    if (o.policy == XBFS_POLICY_SCHEDULE)
+        XB_HIP(hipMemcpyAsync(c->d_sched, o.schedule, (size_t)o.schedule_len,
+                              hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl, wait_ticks(c, o));
     XB_LAUNCHED();
-    XB_HIP(hipMemcpyAsync(h_out, c->d_b64_out, sizeof(long long) * 2, hipMemcpyDeviceToHost,
-                          c->stream));
-    XB_HIP(hipStreamSynchronize(c->stream));
-    long long n_f = h_out[0], m_f = h_out[1], m_u = c->m;
-    uint64_t active = all;
-    int32_t L = 0, max_level = 0;
-    const int32_t H = c->b64_H;
-    while (n_f > 0 && L + 1 < H) {
-        const int mode = b64_mode(o, L, m_f, m_u);
-        a.F_cur = c->d_b64_F + (size_t)(L & 1) * (size_t)n;
-        a.F_next = c->d_b64_F + (size_t)((L + 1) & 1) * (size_t)n;
-        a.vis_cur = c->d_b64_hist + (size_t)L * (size_t)n;
-        a.vis_next = c->d_b64_hist + (size_t)(L + 1) * (size_t)n;
-        if (mode) hipLaunchKernelGGL(k_b64_pull, dim3(grid), dim3(XB_B64_BLOCK), 0, c->stream, a);
-        else hipLaunchKernelGGL(k_b64_push, dim3(grid), dim3(XB_B64_BLOCK), 0, c->stream, a);
-        hipLaunchKernelGGL(k_b64_fin, dim3(grid), dim3(XB_B64_BLOCK), 0, c->stream, a);
-        hipLaunchKernelGGL(k_b64_reduce, dim3(1), dim3(1024), 0, c->stream, c->d_part,
-                           (int64_t)XB_PMAX, grid, XB_B64_F_NF, 4, XB_B64_F_OR, c->d_b64_out);
-        XB_LAUNCHED();
-        XB_HIP(hipMemcpyAsync(h_out, c->d_b64_out, sizeof(long long) * 4, hipMemcpyDeviceToHost,
-                              c->stream));
-        XB_HIP(hipStreamSynchronize(c->stream));
-        n_f = h_out[0];
-        m_f = h_out[1];
-        m_u = h_out[2];
-        active = (uint64_t)h_out[3];
-        L++;
-        if (n_f > 0) max_level = L;
+    XB_HIP(hipEventRecord(c->ev_b64[0], c->stream));
+    void *kargs[] = {&a};
+    XB_HIP(hipLaunchCooperativeKernel((const void *)k_b64_persistent, dim3(grid), dim3(XB_B64_BLOCK),
+                                      kargs, 0, c->stream));
+    XB_HIP(hipEventRecord(c->ev_b64[1], c->stream));
+    XB_HIP(hipEventSynchronize(c->ev_b64[1]));
+    float ms = 0.0f;
+    XB_HIP(hipEventElapsedTime(&ms, c->ev_b64[0], c->ev_b64[1]));
+    XB_HIP(hipMemcpy(c->h_ctl, c->d_ctl, sizeof(XbCtl), hipMemcpyDeviceToHost));
+    if (c->h_ctl->error) return XBFS_ERR_DEVICE;
+    long long h_out[XB_B64_OUT];
+    XB_HIP(hipMemcpy(h_out, c->d_b64_out, sizeof(h_out), hipMemcpyDeviceToHost));
+    const int32_t L = (int32_t)h_out[0];
+    // per-direction device time from the kernel's wall-clock stamps
+    double td_s = 0, bu_s = 0;
+    for (int32_t l = 0; l < L; l++) {
+        const double t = (double)(h_out[8 + l + 1] - h_out[8 + l]) / c->wall_hz;
+        if (h_out[8 + XB_B64_HMAX + 2 + l]) bu_s += t;
+        else td_s += t;
     }
-    const int32_t nslots = L + 1;
-    const uint64_t unfinished = n_f > 0 ? (active & all) : 0ull;
-    const uint64_t done = all & ~unfinished;
-    hipLaunchKernelGGL(k_b64_flush, dim3(grid), dim3(XB_B64_BLOCK), 0, c->stream, c->d_b64_hist,
-                       nslots, n, nsrc, done, c->d_b64_deg, c->d_b64_lv, c->d_part,
-                       (int64_t)XB_PMAX);
-    hipLaunchKernelGGL(k_b64_reduce, dim3(1), dim3(1024), 0, c->stream, c->d_part,
-                       (int64_t)XB_PMAX, grid, XB_B64_F_VC, 2, -1, c->d_b64_out);
-    XB_LAUNCHED();
-    XB_HIP(hipMemcpyAsync(h_out, c->d_b64_out, sizeof(long long) * 2, hipMemcpyDeviceToHost,
-                          c->stream));
-    XB_HIP(hipStreamSynchronize(c->stream));
-    long long v_cc = h_out[0], m_cc = h_out[1];
+    const double rows_s = (double)(h_out[8 + XB_B64_HMAX + 1] - h_out[8 + L]) / c->wall_hz;
+    // XBFS_B64_TRACE=1: per-level device times of the batch on stderr
+    if (const char *tr = getenv("XBFS_B64_TRACE")) {
+        if (tr[0] == '1') {
+            for (int32_t l = 0; l < L; l++)
+                fprintf(stderr, "[b64] level %d %s %.1f us\n", l,
+                        h_out[8 + XB_B64_HMAX + 2 + l] ? "pull" : "push",
+                        (double)(h_out[8 + l + 1] - h_out[8 + l]) / c->wall_hz * 1e6);
+            fprintf(stderr, "[b64] rows pass 1 %.1f us, pass 2 %.1f us\n",
+                    (double)(h_out[8 + XB_B64_HMAX] - h_out[8 + L]) / c->wall_hz * 1e6,
+                    (double)(h_out[8 + XB_B64_HMAX + 1] - h_out[8 + XB_B64_HMAX]) / c->wall_hz * 1e6);
+        }
+    }
+    const uint64_t unfinished = (uint64_t)h_out[1];
+    int32_t max_level = h_out[2] > 0 ? L : std::max(L - 1, 0);
+    long long v_cc = h_out[3], m_cc = h_out[4];
     // sources whose traversal outlasts the history window: single-source engine
     for (int32_t s = 0; s < nsrc; s++) {
         if (!((unfinished >> s) & 1ull)) continue;
@@ -940,10 +1019,7 @@ int xbfs_bfs_batch64(xbfs_ctx *c, const int32_t *srcs, int32_t nsrc, const xbfs_
         m_cc += s1.m_cc;
         max_level = std::max(max_level, s1.max_level);
     }
-    XB_HIP(hipEventRecord(c->ev_b64[1], c->stream));
-    XB_HIP(hipEventSynchronize(c->ev_b64[1]));
-    float ms = 0.0f;
-    XB_HIP(hipEventElapsedTime(&ms, c->ev_b64[0], c->ev_b64[1]));
+    XB_HIP(hipStreamSynchronize(c->stream));
     if (levels_out)
         XB_HIP(hipMemcpy(levels_out, c->d_b64_lv, sizeof(int32_t) * (size_t)n * (size_t)nsrc,
                          hipMemcpyDeviceToHost));
@@ -954,6 +1030,9 @@ int xbfs_bfs_batch64(xbfs_ctx *c, const int32_t *srcs, int32_t nsrc, const xbfs_
         st->max_level = max_level;
         st->v_cc = v_cc;
         st->m_cc = m_cc;
+        st->td_s = td_s;
+        st->bu_s = bu_s;
+        st->gen_s = rows_s;
     }
     return XBFS_OK;
 }

@@ -66,18 +66,25 @@ struct xbfs_ctx {
     // multi-source seeds (grown on demand)
     int32_t *d_srcs = nullptr;
     int64_t srcs_cap = 0;
-    // 64-source batch engine (allocated on the first xbfs_bfs_batch64)
+    // 64-source batch engine (allocated on the first xbfs_bfs_batch64): the
+    // graph relabelled by degree (rank order) and the batch state
     int b64_ready = 0;
     int32_t b64_H = 0;             // history slots
-    int64_t b64_nseg = 0;          // 32-edge segments
-    int32_t *d_b64_deg = nullptr;  // int32[n]
-    int32_t *d_b64_segu = nullptr; // int32[nseg]
-    int64_t *d_b64_sege = nullptr; // int64[nseg+1]
-    uint64_t *d_b64_F = nullptr;   // 2 x uint64[n]
+    int32_t b64_grid = 0;          // co-resident blocks of k_b64_persistent
+    int64_t b64_nseg = 0;          // 32-edge segments of the relabelled rows
+    int32_t *d_b64_rank = nullptr; // original id -> rank
+    int32_t *d_b64_ord = nullptr;  // rank -> original id
+    int32_t *d_b64_rdeg = nullptr; // degree by rank
+    int64_t *d_b64_rbeg = nullptr; // int64[n+1]
+    int32_t *d_b64_radj = nullptr; // int32[m]
+    int32_t *d_b64_segr = nullptr; // int32[nseg]: rank owning segment i
+    int64_t *d_b64_sege = nullptr; // int64[nseg+1]: first edge of segment i
+    uint64_t *d_b64_F = nullptr;   // H x uint64[n]
+    uint64_t *d_b64_vis = nullptr; // uint64[n]
     uint64_t *d_b64_acc = nullptr; // uint64[n]
-    uint64_t *d_b64_hist = nullptr;// H x uint64[n]
+    uint64_t *d_b64_rec = nullptr; // 8 x uint64[n]
     int32_t *d_b64_lv = nullptr;   // 64 x int32[n]
-    long long *d_b64_out = nullptr;// 8 reduced statistics
+    long long *d_b64_out = nullptr;// 8 results
     int32_t *d_b64_srcs = nullptr; // 64 sources
     hipEvent_t ev_b64[2] = {nullptr, nullptr};
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;

@@ -13,6 +13,13 @@
 #include "xbfs_device.h"
 #include "xbfs_kernels.h"
 
+// p[0..k) = 0, k <= 64 (small counters; a kernel, not a memset, so the same
+// call is valid inside a captured graph)
+__global__ void k_zero_u64(unsigned long long *p, int k)
+{
+    if ((int)threadIdx.x < k) p[threadIdx.x] = 0ull;
+}
+
 // nz bitmap: bit v set iff deg(v) > 0.  One wave per 64-vertex word.
 __global__ void __launch_bounds__(256) k_build_nz(const int64_t *__restrict__ beg, int64_t n,
                                                   uint64_t *__restrict__ nz, int64_t nwords)

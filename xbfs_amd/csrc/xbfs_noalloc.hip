@@ -102,11 +102,6 @@ __global__ void __launch_bounds__(256) k_na_degree_info(const int64_t *__restric
     }
 }
 
-__global__ void k_na_zero_u64(unsigned long long *p, int k)
-{
-    if ((int)threadIdx.x < k) p[threadIdx.x] = 0ull;
-}
-
 extern "C" int xbfs_bfs_noalloc(int device, const int64_t *d_beg, const int32_t *d_adj, int64_t n,
                                 int64_t m, int32_t src, const xbfs_opts *opts,
                                 const float *policy_weights, int32_t *d_level, int32_t *d_parent,
@@ -139,7 +134,7 @@ extern "C" int xbfs_bfs_noalloc(int device, const int64_t *d_beg, const int32_t 
     const int64_t nwords = ceil_div(n, 64);
 
     // per-graph tables: degree bitmap, hub table, (count, largest degree)
-    hipLaunchKernelGGL(k_na_zero_u64, dim3(1), dim3(64), 0, s0, cnt, 2);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s0, cnt, 2);
     hipLaunchKernelGGL(k_build_nz, dim3(grid_for(nwords * 64, 256, 65536)), dim3(256), 0, s0, d_beg,
                        n, nz, nwords);
     hipLaunchKernelGGL(k_build_hd, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s0, d_beg, d_adj, n,
