@@ -12,8 +12,10 @@
  *   xbfs_graph_write
  *       the output side of CSRGenerator/tuple_text_to_bin.cpp:359-360, but in
  *       the int32 adjacency layout the loader reads (bfs_gpu_update.cu:52).
- *   xbfs_graph_from_edge_text
+ *   xbfs_graph_from_tuples (and the text_to_bin command line)
  *       CSRGenerator/tuple_text_to_bin.cpp:59-426 (text edge list -> CSR).
+ *   xbfs_graph_from_edge_text
+ *       no reference counterpart: a deduplicated, loop-free edge-list reader.
  *   xbfs_gen_rmat / xbfs_gen_er / xbfs_gen_grid
  *       no reference counterpart (the reference ships no generator); seeded
  *       inputs for the benchmark configurations.
@@ -77,12 +79,22 @@ typedef struct {
  * (the reference then prints "Weight file cannot open" and continues). */
 int xbfs_graph_load(const char *beg_file, const char *csr_file,
                     const char *weight_file, xbfs_host_graph *g);
-/* Writes <prefix>_beg_pos.bin and <prefix>_csr.bin. */
+/* Writes <prefix>_beg_pos.bin and <prefix>_csr.bin, and <prefix>_weight.bin
+ * (int32 per entry) when g->weight is set. */
 int xbfs_graph_write(const char *prefix, const xbfs_host_graph *g);
 /* Reads a whitespace-separated "src dst [weight]" text edge list (lines
  * starting with '#' or '%' skipped).  reverse != 0 also inserts dst->src.
- * Self-loops dropped, rows sorted and deduplicated. */
+ * Self-loops dropped, rows sorted and deduplicated: a clean undirected graph
+ * for traversal (no reference counterpart; compare xbfs_graph_from_tuples). */
 int xbfs_graph_from_edge_text(const char *path, int reverse, xbfs_host_graph *g);
+/* The reference converter's semantics, CSRGenerator/tuple_text_to_bin.cpp:59-426:
+ * skip_lines header lines, then whitespace-separated ids read as (src, dst)
+ * pairs; n = largest id + 1; rows in file order, duplicates and self-loops
+ * kept; reverse != 0 also appends src to dst's row (m = 2 x pairs, else m =
+ * pairs); a weight 1..63 per pair in g->weight.  Written by xbfs_graph_write
+ * as int32 adjacency and weights, the layout the loader reads (the reference
+ * writes int64, which its own int32 loader cannot read). */
+int xbfs_graph_from_tuples(const char *path, int reverse, int64_t skip_lines, xbfs_host_graph *g);
 void xbfs_graph_free(xbfs_host_graph *g);
 
 /* RMAT/Kronecker generator: 2^scale vertices, ef*2^scale sampled edges with
@@ -134,7 +146,8 @@ int64_t xbfs_num_edges(const xbfs_ctx *ctx);
 
 typedef struct {
     int32_t block_dim;      /* accepted for CLI compatibility (64..1024, power of
-                               two); the level phases always run 256-thread blocks */
+                               two); the persistent kernel runs 1024-thread blocks,
+                               the host-driven phases 256-thread blocks           */
     int32_t grid_dim;       /* blocks; 0 = sized to the GPU (persistent: all
                                co-resident blocks, a smaller value caps it)        */
     float alpha;            /* frontier-edge ratio that switches to bottom-up       */

@@ -132,6 +132,26 @@ int main(int argc, char **argv)
     EXPECT(rc == XBFS_OK || rc == XBFS_ERR_FORMAT);
     EXPECT(xbfs_graph_from_edge_text((dir + "/missing.txt").c_str(), 1, &t) == XBFS_ERR_IO);
 
+    // reference-semantics tuples: header lines, duplicates, self-loops kept,
+    // a lone trailing id, no final newline; then written with weights and loaded
+    const char *tup = "h1\nh2\n0 1\n1\t2\n0 1\n2 2\r\n3 0\n7";
+    write_bytes(dir + "/tuples.txt", tup, std::strlen(tup));
+    EXPECT(xbfs_graph_from_tuples((dir + "/tuples.txt").c_str(), 1, 2, &t) == XBFS_OK);
+    EXPECT(t.n == 8 && t.m == 10 && t.weight != nullptr);
+    EXPECT(xbfs_check_csr(t.n, t.m, t.beg_pos, t.adj) == XBFS_OK);
+    EXPECT(xbfs_graph_write((dir + "/tw").c_str(), &t) == XBFS_OK);
+    xbfs_graph_free(&t);
+    EXPECT(xbfs_graph_load((dir + "/tw_beg_pos.bin").c_str(), (dir + "/tw_csr.bin").c_str(),
+                           (dir + "/tw_weight.bin").c_str(), &t) == XBFS_OK);
+    EXPECT(t.m == 10 && t.weight != nullptr);
+    xbfs_graph_free(&t);
+    EXPECT(xbfs_graph_from_tuples((dir + "/tuples.txt").c_str(), 0, 9, &t) == XBFS_ERR_FORMAT);
+    EXPECT(xbfs_graph_from_tuples((dir + "/junk.txt").c_str(), 0, 0, &t) == XBFS_ERR_FORMAT);
+    EXPECT(xbfs_graph_from_tuples((dir + "/missing.txt").c_str(), 0, 0, &t) == XBFS_ERR_IO);
+    const char *empty = "only a header\n";
+    write_bytes(dir + "/empty.txt", empty, std::strlen(empty));
+    EXPECT(xbfs_graph_from_tuples((dir + "/empty.txt").c_str(), 0, 1, &t) == XBFS_ERR_FORMAT);
+
     xbfs_graph_free(&r);
     xbfs_graph_free(&e);
     xbfs_graph_free(&gr);

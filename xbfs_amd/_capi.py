@@ -85,6 +85,7 @@ SIGNATURES = {
     "xbfs_graph_load": (c_int, [c_char_p, c_char_p, c_char_p, POINTER(HostGraph)]),
     "xbfs_graph_write": (c_int, [c_char_p, POINTER(HostGraph)]),
     "xbfs_graph_from_edge_text": (c_int, [c_char_p, c_int, POINTER(HostGraph)]),
+    "xbfs_graph_from_tuples": (c_int, [c_char_p, c_int, c_int64, POINTER(HostGraph)]),
     "xbfs_graph_free": (None, [POINTER(HostGraph)]),
     "xbfs_gen_rmat": (c_int, [c_int32, c_int32, c_double, c_double, c_double, c_uint64,
                               c_int32, POINTER(HostGraph)]),
@@ -196,6 +197,17 @@ class Graph:
                                     weight_file.encode() if weight_file else None, byref(hg)),
               "graph_load")
         return Graph._from_host(hg)
+
+    @staticmethod
+    def from_tuples(path, reverse=True, skip_lines=0):
+        """The reference converter's semantics (xbfs_graph_from_tuples).
+        Returns (graph, weights int8[m])."""
+        hg = HostGraph()
+        check(lib().xbfs_graph_from_tuples(path.encode(), int(reverse), int(skip_lines), byref(hg)),
+              "graph_from_tuples")
+        m = hg.m
+        w = np.ctypeslib.as_array(hg.weight, shape=(m,)).copy() if m > 0 else np.zeros(0, np.int8)
+        return Graph._from_host(hg), w
 
     @staticmethod
     def from_edge_text(path, reverse=True):

@@ -334,7 +334,111 @@ int xbfs_graph_write(const char *prefix, const xbfs_host_graph *g)
     if (!f) return XBFS_ERR_IO;
     ok = write_all(f, g->adj, g->m);
     if (fclose(f) != 0) ok = false;
+    if (!ok) return XBFS_ERR_IO;
+    if (g->weight) {
+        // int32 per entry, the layout the loader reads and narrows to int8
+        std::vector<int32_t> w((size_t)g->m);
+        for (int64_t j = 0; j < g->m; j++) w[(size_t)j] = g->weight[j];
+        f = fopen((p + "_weight.bin").c_str(), "wb");
+        if (!f) return XBFS_ERR_IO;
+        ok = write_all(f, w.data(), g->m);
+        if (fclose(f) != 0) ok = false;
+    }
     return ok ? XBFS_OK : XBFS_ERR_IO;
+}
+
+// Tuple text -> CSR with the reference converter's semantics
+// (CSRGenerator/tuple_text_to_bin.cpp:59-426): after `skip_lines` header
+// lines the file is a stream of whitespace-separated vertex ids read as
+// (src, dst) pairs (:129-170; a lone last id still counts toward the largest
+// id); n = largest id + 1 (:163); the degree pass counts src, and dst too when
+// reverse (:250-281); the fill pass appends dst to src's row in file order and,
+// when reverse, src to dst's row (a self-loop then appears twice in its row,
+// :297-313).  Duplicates and self-loops are kept.  m = pairs, or twice that
+// when reverse.  Weights: one per pair, 1..63 (:316-320), from a fixed-seed
+// generator of our own, the same on both entries of a reversed pair.
+// Beyond the reference: ids must be non-negative decimal integers below
+// 2^31 - 1 (the reference's atol accepts anything), skipping more lines than
+// the file holds or a file with no id is XBFS_ERR_FORMAT, and a newline as
+// the very first byte counts as a line (the reference's skip loop starts
+// looking at the second byte, :89-105).
+int xbfs_graph_from_tuples(const char *path, int reverse, int64_t skip_lines, xbfs_host_graph *g)
+{
+    if (!path || !g || skip_lines < 0) return XBFS_ERR_ARG;
+    std::memset(g, 0, sizeof(*g));
+    const int64_t fsz = file_size(path);
+    if (fsz < 0) return XBFS_ERR_IO;
+    std::string text((size_t)fsz, '\0');
+    FILE *f = fopen(path, "rb");
+    if (!f) return XBFS_ERR_IO;
+    const bool rd = fsz == 0 || fread(&text[0], 1, (size_t)fsz, f) == (size_t)fsz;
+    fclose(f);
+    if (!rd) return XBFS_ERR_IO;
+    size_t pos = 0;
+    for (int64_t k = 0; k < skip_lines; k++) {
+        const size_t nl = text.find('\n', pos);
+        if (nl == std::string::npos) return XBFS_ERR_FORMAT;
+        pos = nl + 1;
+    }
+    // pass 1: the ids
+    std::vector<int32_t> ids;
+    int64_t vmax = -1;
+    auto is_ws = [](char ch) { return ch == ' ' || ch == '\t' || ch == '\n' || ch == '\r'; };
+    while (pos < text.size()) {
+        while (pos < text.size() && is_ws(text[pos])) pos++;
+        if (pos >= text.size()) break;
+        int64_t v = 0;
+        size_t q = pos;
+        while (q < text.size() && !is_ws(text[q])) {
+            const char ch = text[q];
+            if (ch < '0' || ch > '9') return XBFS_ERR_FORMAT;
+            v = v * 10 + (ch - '0');
+            if (v > INT32_MAX - 1) return XBFS_ERR_FORMAT;
+            q++;
+        }
+        ids.push_back((int32_t)v);
+        vmax = std::max(vmax, v);
+        pos = q;
+    }
+    if (vmax < 0) return XBFS_ERR_FORMAT;
+    const int64_t n = vmax + 1, pairs = (int64_t)ids.size() / 2;
+    const int64_t m = reverse ? 2 * pairs : pairs;
+    int64_t *beg = alloc_array<int64_t>(n + 1);
+    int32_t *adj = alloc_array<int32_t>(m > 0 ? m : 1);
+    int8_t *w = alloc_array<int8_t>(m > 0 ? m : 1);
+    std::vector<int64_t> cur;
+    if (!beg || !adj || !w) { free(beg); free(adj); free(w); return XBFS_ERR_NOMEM; }
+    // pass 2: degrees, offsets
+    for (int64_t i = 0; i <= n; i++) beg[i] = 0;
+    for (int64_t e = 0; e < pairs; e++) {
+        beg[ids[(size_t)(2 * e)] + 1]++;
+        if (reverse) beg[ids[(size_t)(2 * e + 1)] + 1]++;
+    }
+    for (int64_t i = 0; i < n; i++) beg[i + 1] += beg[i];
+    // pass 3: rows in file order
+    cur.assign(beg, beg + n);
+    uint64_t rs = 0x9e3779b97f4a7c15ull;
+    for (int64_t e = 0; e < pairs; e++) {
+        const int32_t a = ids[(size_t)(2 * e)], b = ids[(size_t)(2 * e + 1)];
+        rs += 0x9e3779b97f4a7c15ull; // splitmix64
+        uint64_t z = rs;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        const int8_t wt = (int8_t)(z % 63 + 1);
+        w[cur[(size_t)a]] = wt;
+        adj[cur[(size_t)a]++] = b;
+        if (reverse) {
+            w[cur[(size_t)b]] = wt;
+            adj[cur[(size_t)b]++] = a;
+        }
+    }
+    g->n = n;
+    g->m = m;
+    g->beg_pos = beg;
+    g->adj = adj;
+    g->weight = w;
+    return XBFS_OK;
 }
 
 int xbfs_gen_rmat(int32_t scale, int32_t ef, double a, double b, double c,
