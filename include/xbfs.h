@@ -165,9 +165,6 @@ typedef struct {
     int32_t max_levels;     /* > 0: stop with XBFS_ERR_LIMIT at this many levels     */
     int32_t wait_limit_us;  /* > 0: bound of one grid-barrier wait in the persistent
                                kernel (default 4 s); past it XBFS_ERR_DEVICE          */
-    int32_t single_block;   /* persistent kernel: 0 (default) = scan-free levels of at
-                               most 2048 frontier edges run on one workgroup, level
-                               after level, without grid barriers; -1 = never          */
 } xbfs_opts;
 
 void xbfs_opts_default(xbfs_opts *o);

@@ -269,17 +269,13 @@ def test_launch_shapes_and_bins(kw):
 def test_small_frontier_block_counts(grid_dim):
     """Persistent grids around the small-frontier block count (32 active
     blocks): at 32 small levels stay on the whole grid, at 33 one block parks
-    and rejoins on every level that needs it, at 48 sixteen do.  Run with the
-    single-block levels too (they take over the smallest frontiers) and
-    without them (single_block = -1), so every small level runs on the 32
-    blocks."""
+    and rejoins on every level that needs it, at 48 sixteen do."""
     for g in (Graph.grid(64, 64), Graph.rmat(12, 16, seed=1012)):
         ctx = Context(g)
         for s in seeded_sources(g, 3):
             for pol in POLICIES[:4]:
-                for sb in (0, -1):
-                    say("small blocks", grid_dim, g.n, s, pol, sb)
-                    run_check(ctx, g, s, engines=(1,), grid_dim=grid_dim, single_block=sb, **pol)
+                say("small blocks", grid_dim, g.n, s, pol)
+                run_check(ctx, g, s, grid_dim=grid_dim, **pol)
         ctx.close()
 
 
@@ -377,15 +373,14 @@ def test_config3b_grid4096_full_size():
     g = Graph.grid(R, C)
     ctx = Context(g)
     rr, cc = np.divmod(np.arange(g.n, dtype=np.int64), C)
-    for s, eng, sb in ((0, 1, 0), (R * C - 1, 0, 0), ((R // 2) * C + C // 3, 1, 0),
-                       ((R // 2) * C + C // 3, 1, -1)):
-        say("grid4096", s, "persistent", eng, "single_block", sb)
-        lv, _, _, st = ctx.bfs(s, persistent=eng, single_block=sb)
+    for s, eng in ((0, 1), (R * C - 1, 0), ((R // 2) * C + C // 3, 1)):
+        say("grid4096", s, "persistent", eng)
+        lv, _, _, st = ctx.bfs(s, persistent=eng)
         sr, sc = divmod(s, C)
         want = (np.abs(rr - sr) + np.abs(cc - sc)).astype(np.int32)
         assert (lv == want).all()
         assert st["max_level"] == want.max()
-        print(f"[grid4096] src {s} persistent {eng} single_block {sb}: {st['n_levels']} levels, "
+        print(f"[grid4096] src {s} persistent {eng}: {st['n_levels']} levels, "
               f"{st['t_bfs_s'] * 1e3:.1f} ms, {st['t_bfs_s'] / max(st['n_levels'], 1) * 1e6:.1f} "
               f"us/level", flush=True)
     ctx.close()

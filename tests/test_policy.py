@@ -150,8 +150,10 @@ def test_persistent_decisions_follow_the_model():
 @pytest.mark.gpu
 def test_trained_policy_gain_on_held_out_graph():
     """The committed weights (trained by xbfs_amd/train_policy.py on RMAT
-    scale 18-22) against the heuristic on a held-out graph: reports the
-    end-to-end speedup and requires the learned policy not to be slower."""
+    scale 18-22) against the heuristic on a held-out graph (seed 777, outside
+    the training set): the end-to-end time over 8 sources, each the median of
+    5 traversals, must be at least 1.1x shorter, the target the learned
+    policy is built for."""
     if not os.path.exists(P.WEIGHTS):
         pytest.skip("no trained weights committed yet")
     g = Graph.rmat(20, 16, seed=777)
@@ -162,7 +164,7 @@ def test_trained_policy_gain_on_held_out_graph():
     print(f"[policy] held-out RMAT-20: heuristic {tot['heuristic']*1e3:.3f} ms, learned "
           f"{tot['learned']*1e3:.3f} ms, oracle-best {tot['oracle_best']*1e3:.3f} ms, "
           f"gain {gain:.3f}x", flush=True)
-    assert gain >= 0.97
+    assert gain >= 1.1
 
 
 @pytest.mark.gpu

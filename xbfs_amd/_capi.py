@@ -50,8 +50,7 @@ class Opts(Structure):
                 ("bin_thread_max", c_int32), ("bin_wave_max", c_int32),
                 ("schedule", POINTER(c_int8)), ("schedule_len", c_int32),
                 ("time_levels", c_int32), ("bin_block_max", c_int32),
-                ("max_levels", c_int32), ("wait_limit_us", c_int32),
-                ("single_block", c_int32)]
+                ("max_levels", c_int32), ("wait_limit_us", c_int32)]
 
 
 class LevelRec(Structure):

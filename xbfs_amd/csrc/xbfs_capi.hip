@@ -25,6 +25,7 @@
 
 // dynamic LDS: one XbWaveLds per wave of the block
 #define XB_LDS_PHASE (sizeof(XbWaveLds) * (XB_PBLOCK / 64))
+#define XB_LDS_PERSIST (sizeof(XbWaveLds) * (XB_PERSIST_BLOCK / 64))
 
 #define XB_HIP(expr)                                                                   \
     do {                                                                               \
@@ -104,7 +105,6 @@ void xbfs_opts_default(xbfs_opts *o)
     o->bin_block_max = XB_DEFAULT_T3;
     o->max_levels = 0;
     o->wait_limit_us = 0;
-    o->single_block = 0;
 }
 
 int64_t xbfs_num_vertices(const xbfs_ctx *c) { return c ? c->n : -1; }
@@ -300,7 +300,6 @@ int xb_resolve_opts(const xbfs_opts *in, xbfs_opts *o)
     }
     if (!(o->alpha >= 0.0f)) return XBFS_ERR_ARG;
     if (o->max_levels < 0 || o->wait_limit_us < 0) return XBFS_ERR_ARG;
-    if (o->single_block != 0 && o->single_block != -1) return XBFS_ERR_ARG;
     return XBFS_OK;
 }
 
@@ -361,7 +360,6 @@ static XbArgs make_args(xbfs_ctx *c, int32_t src, const xbfs_opts &o)
     a.alpha = o.alpha;
     a.max_levels = (int32_t)std::min<int64_t>(c->n + 1, INT32_MAX - 4);
     if (o.max_levels > 0) a.max_levels = std::min(a.max_levels, o.max_levels);
-    a.solo = o.single_block == 0 ? 1 : 0;
     return a;
 }
 
@@ -1148,7 +1146,7 @@ static hipError_t launch_coop_level(xbfs_ctx *c, const XbArgs &a, int32_t L, int
     XbArgs args = a;
     void *kargs[] = {&args, &L, &mode, &q_valid, &need_sweep, &enqueue, &extra_sweep};
     return hipLaunchCooperativeKernel((const void *)k_coop_level, dim3(c->coop_blocks),
-                                      dim3(XB_PERSIST_BLOCK), kargs, XB_LDS_PHASES, c->stream);
+                                      dim3(XB_PERSIST_BLOCK), kargs, XB_LDS_PERSIST, c->stream);
 }
 
 extern "C" int xbfs_bfs_probe(xbfs_ctx *c, int32_t src, const xbfs_opts *opts, int32_t follow_best,

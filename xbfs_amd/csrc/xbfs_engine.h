@@ -109,7 +109,6 @@ struct XbArgs {
     int32_t policy;
     float alpha;
     int32_t max_levels;
-    int32_t solo;       // 1: small scan-free levels may run on one block (k_persistent)
 };
 
 __host__ __device__ inline int32_t *xb_q(const XbArgs &a, int b, int k)
@@ -121,13 +120,16 @@ __host__ __device__ inline uint64_t *xb_f(const XbArgs &a, int32_t level)
     return a.fbm + (int64_t)(level & 3) * a.fstride;
 }
 
-// Block-wide LDS shared by the phases.
+// Block-wide LDS shared by the phases (static).
 union XbSmem {
     struct {
         long long pref[XB_GCAP + 1]; // exclusive degree prefix of the vertices of one pass
         long long b[XB_GCAP];        // their row starts
         int32_t u[XB_GCAP];
     } td;
+    struct {
+        float x[8], y[4];
+    } mlp;
 };
 
 #define XB_CHUNK_WORDS 16 /* bitmap words per wave-owned chunk: 1024 vertices, one 128-B line */
@@ -138,34 +140,6 @@ struct XbWaveLds {
     unsigned long long f1[XB_CHUNK_WORDS];  // bottom-up: new level-(L+1) bits
     unsigned long long va[XB_CHUNK_WORDS];  // bottom-up: new visited bits
 };
-
-// Single-block levels (persistent kernel): a scan-free top-down level whose
-// queue is exact and whose frontier has at most XB_SOLO_MF edges runs on
-// block 0 alone, which then keeps going level after level with no grid
-// barrier while every other block parks.  One CU issues the level's scattered
-// loads and atomics at about 5 ns per edge, while a level on many blocks
-// costs 12-19 us whatever its size, so the cap is 2048 edges (2 per thread of
-// the 1024-thread block).  A level discovers at most m_f vertices, so the
-// lists never overflow.  They live in LDS (two buffers: the level being
-// expanded and the next one), overlaying the phases' LDS, which no
-// single-block level uses.
-#ifndef XB_SOLO_MF
-#define XB_SOLO_MF 2048
-#endif
-#define XB_SOLO_NF XB_SOLO_MF
-#define XB_SOLO_K (XB_SOLO_MF / XB_PERSIST_BLOCK)
-struct XbSolo {
-    long long b[2][XB_SOLO_NF]; // row starts
-    int32_t v[2][XB_SOLO_NF];   // vertex ids
-    int32_t d[2][XB_SOLO_NF];   // degrees
-    int32_t pf[XB_SOLO_NF + 4]; // exclusive degree prefix of the list being expanded
-};
-
-// Dynamic LDS of k_persistent: XbSmem, then one XbWaveLds per wave; the
-// single-block levels overlay the whole region with XbSolo.
-#define XB_LDS_WAVES_OFF ((sizeof(XbSmem) + 15) / 16 * 16)
-#define XB_LDS_PHASES (XB_LDS_WAVES_OFF + sizeof(XbWaveLds) * (XB_PERSIST_BLOCK / 64))
-#define XB_LDS_PERSIST (XB_LDS_PHASES > sizeof(XbSolo) ? XB_LDS_PHASES : sizeof(XbSolo))
 
 // Loop state every block (or the host) tracks identically.
 struct XbLoop {
@@ -183,8 +157,6 @@ struct XbLoopShared {
     XbLoop st;
     long long n_f, m_f, n_u, m_u, p2_next;
     int32_t open, mode, htag, used, small;
-    int32_t solo;     // the level runs on block 0 alone (XbSolo)
-    int32_t fix_ring; // after single-block levels: bitmap ring (L+3)&3 must be cleared whole
 };
 
 // Small-frontier levels: a scan-free top-down level whose queue is exact and

@@ -100,7 +100,7 @@ __device__ void xb_part_add(const XbArgs &a, int bid, unsigned long long (&v)[K]
 // lane are in flight before the first wait: one memory round trip per 256
 // blocks instead of one per column.
 template <int K>
-__device__ __forceinline__ void xb_part_sum(const XbArgs &a, const int (&f)[K], long long (&out)[K], int ncols,
+__device__ void xb_part_sum(const XbArgs &a, const int (&f)[K], long long (&out)[K], int ncols,
                             int wave = 0)
 {
     if ((int)(threadIdx.x >> 6) != wave) return;
@@ -131,7 +131,7 @@ __device__ __forceinline__ void xb_part_sum(const XbArgs &a, const int (&f)[K], 
 // ctl->error and returns false so every block leaves the kernel.  With watch_hb, progress also means a change of ctl->heartbeat
 // (blocks parked during small-frontier levels wait as long as those levels
 // keep coming).
-__device__ __forceinline__ bool xb_barrier(XbCtl *c, uint32_t *cnt, uint32_t *genw, uint32_t members, bool watch_hb)
+__device__ bool xb_barrier(XbCtl *c, uint32_t *cnt, uint32_t *genw, uint32_t members, bool watch_hb)
 {
     __shared__ int s_ok;
     __syncthreads();
@@ -191,14 +191,14 @@ __device__ __forceinline__ bool xb_barrier(XbCtl *c, uint32_t *cnt, uint32_t *ge
 }
 
 // Grid-wide barrier (every block of the launch).
-__device__ __forceinline__ bool xb_grid_sync(XbCtl *c)
+__device__ bool xb_grid_sync(XbCtl *c)
 {
     return xb_barrier(c, &c->bar_count, &c->bar_gen, gridDim.x, false);
 }
 
 // Barrier of the blocks taking part in the current level: the whole grid, or
 // the first nact blocks during small-frontier levels.
-__device__ __forceinline__ bool xb_level_sync(XbCtl *c, int nact)
+__device__ bool xb_level_sync(XbCtl *c, int nact)
 {
     if (nact == (int)gridDim.x) return xb_grid_sync(c);
     return xb_barrier(c, &c->sbar_count, &c->sbar_gen, (uint32_t)nact, false);
@@ -206,7 +206,7 @@ __device__ __forceinline__ bool xb_level_sync(XbCtl *c, int nact)
 
 // A block sitting out small-frontier levels: arrives at the grid barrier and
 // waits there until the active blocks arrive too.
-__device__ __forceinline__ bool xb_park(XbCtl *c)
+__device__ bool xb_park(XbCtl *c)
 {
     return xb_barrier(c, &c->bar_count, &c->bar_gen, gridDim.x, true);
 }
@@ -214,7 +214,7 @@ __device__ __forceinline__ bool xb_park(XbCtl *c)
 // ---------------------------------------------------------------------------
 // init
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void ph_init(const XbArgs &a, int bid, int nb)
+__device__ void ph_init(const XbArgs &a, int bid, int nb)
 {
     const int64_t gtid = (int64_t)bid * blockDim.x + threadIdx.x;
     const int64_t nthr = (int64_t)nb * blockDim.x;
@@ -281,7 +281,7 @@ __device__ __forceinline__ void ph_init(const XbArgs &a, int bid, int nb)
 // (they are reached but are not frontier vertices).  Duplicates are resolved
 // by the visited bit: only the lane that sets it writes the seed.  The
 // level-0 queue is then built by a sweep of F_0.
-__device__ __forceinline__ void ph_seed(const XbArgs &a, int bid, int nb)
+__device__ void ph_seed(const XbArgs &a, int bid, int nb)
 {
     const int64_t gtid = (int64_t)bid * blockDim.x + threadIdx.x;
     const int64_t nthr = (int64_t)nb * blockDim.x;
@@ -315,7 +315,7 @@ __device__ __forceinline__ void ph_seed(const XbArgs &a, int bid, int nb)
 // (L+3)&3 = (L-1)&3, which no phase of level L reads or writes (nor the
 // level's statistics and decision, so the two may run side by side).
 // Threads below t_lo of each block take no part.
-__device__ __forceinline__ void ph_level_prep(const XbArgs &a, int32_t L, int bid, int nb, int t_lo = 0)
+__device__ void ph_level_prep(const XbArgs &a, int32_t L, int bid, int nb, int t_lo = 0)
 {
     if ((int)threadIdx.x < t_lo) return;
     const int tid = (int)threadIdx.x - t_lo, bdim = (int)blockDim.x - t_lo;
@@ -386,7 +386,7 @@ __device__ __forceinline__ int xb_chunk_list(uint64_t myword, int64_t w0, XbWave
 // passes: count per wave, reserve once per block and bin (one atomic), then
 // write; the second pass re-reads F_L and the degrees (mostly from L2).
 // With stats, adds (count, degree sum) to sweep slot L&1.
-__device__ __forceinline__ void ph_sweep(const XbArgs &a, int32_t L, int bid, int nb, bool from_levels, bool stats,
+__device__ void ph_sweep(const XbArgs &a, int32_t L, int bid, int nb, bool from_levels, bool stats,
                          XbWaveLds *lds, bool fresh = false)
 {
     __shared__ int s_wc[16][XB_BINS];
@@ -623,7 +623,7 @@ __device__ void td_concat(const XbArgs &a, const XbTdCtx &t, int32_t cnt, Sel se
 // two levels ahead; none is made, so it is zero), so the counts of level L are
 // stored without reading the column back; set to zero here.
 template <bool SF>
-__device__ __forceinline__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue, XbSmem &sm,
+__device__ void ph_td(const XbArgs &a, int32_t L, int bid, int nb, bool enqueue, XbSmem &sm,
                       long long *carry = nullptr)
 {
     XbTdCtx t;
@@ -713,7 +713,7 @@ __device__ __forceinline__ void ph_td(const XbArgs &a, int32_t L, int bid, int n
 // Level (and top-down single-scan level) stores are nontemporal: each entry is
 // written once per traversal and not read back by its writer, so it need not
 // sit dirty in L2 until the level's release writes it back.
-__device__ __forceinline__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *lds,
+__device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *lds,
                       long long *carry = nullptr)
 {
     const uint64_t *F = xb_f(a, L);
@@ -1015,355 +1015,6 @@ __device__ void xb_mlp_wave_forward(const float *p, const float *x, float *y)
     if (o < XB_MLP_NO) y[o] = t3;
 }
 
-// Level bookkeeping and the mode decision of level L, by the lanes of one
-// wave (wave 0).  sv (lane 0's values count) = {frontier vertices, their degree
-// sum, vertices and degree sum already found for level L+1, of which by the
-// +2 rule}.  Fills ls.open, the counts and the decision (mode, htag, used);
-// when the traversal ends here, block 0 records its totals in ctl (and the
-// level-cap error when max_levels stopped it).  mx / my: LDS scratch of the
-// MLP's 8 inputs and 3 outputs.
-__device__ __forceinline__ void xb_decide(const XbArgs &a, XbLoopShared &ls, const float *s_mlp, float *mx,
-                          float *my, int32_t L, const long long (&sv)[5], int bid)
-{
-    XbCtl *c = a.ctl;
-    if (threadIdx.x == 0) {
-        long long n_u = 0, m_u = 0;
-        const bool open = xb_level_open(ls.st, a, sv[0], sv[1], sv[2], sv[3], &n_u, &m_u);
-        ls.open = open ? 1 : 0;
-        ls.n_f = sv[0];
-        ls.m_f = sv[1];
-        ls.n_u = n_u;
-        ls.m_u = m_u;
-        ls.p2_next = sv[4];
-        if (!open || L >= a.max_levels) {
-            if (bid == 0) {
-                const long long pend_n = sv[2], pend_m = sv[3];
-                xb_st(&c->n_levels, L);
-                xb_st(&c->max_level, pend_n > 0 ? L + 1 : (sv[0] > 0 ? L : L - 1));
-                xb_st(&c->v_cc, (long long)(ls.st.cum_n + pend_n));
-                xb_st(&c->m_cc, (long long)(ls.st.cum_m + pend_m));
-                if (open) xb_st(&c->error, 2);
-            }
-            ls.open = 0;
-        }
-    }
-    xb_wave_sync();
-    if (ls.open) {
-        const float *pred = nullptr;
-        if (a.mlp && L <= XBFS_MLP_MAX_LEVEL) {
-            if (threadIdx.x == 0)
-                xbfs_mlp_features(L, ls.n_f, ls.m_f, ls.n_u, ls.m_u, ls.st.prev_mode, mx);
-            xb_wave_sync();
-            xb_mlp_wave_forward(s_mlp, mx, my);
-            xb_wave_sync();
-            pred = my;
-        }
-        if (threadIdx.x == 0) {
-            int32_t htag = 0, used = 0;
-            ls.mode = xbfs_choose_mode(a.policy, a.sched, a.sched_len, pred, &ls.st.hs, L, ls.m_f,
-                                       a.m, a.alpha, &htag, &used);
-            ls.htag = htag;
-            ls.used = used;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// single-block levels (k_persistent, block 0)
-// ---------------------------------------------------------------------------
-// Block 0 waits until the other `others` blocks of the grid have arrived at
-// the grid barrier, where they stay parked until block 0 arrives too.  Their
-// arrivals release their stores of the current level (zeroing included), and
-// the acquire here makes them visible to block 0.  Bounded like xb_barrier.
-__device__ __forceinline__ bool xb_solo_wait_others(XbCtl *c, uint32_t others)
-{
-    __shared__ int s_ok;
-    if (threadIdx.x == 0) {
-        const unsigned long long limit =
-            __hip_atomic_load(&c->wait_ticks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long t0 = wall_clock64();
-        int ok = 1;
-        while (__hip_atomic_load(&c->bar_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != others) {
-            __builtin_amdgcn_s_sleep(1);
-            if (__hip_atomic_load(&c->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-                ok = 0;
-                break;
-            }
-            if (wall_clock64() - t0 > limit) {
-                int32_t none = 0;
-                __hip_atomic_compare_exchange_strong(&c->error, &none, 1, __ATOMIC_RELAXED,
-                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = 0;
-                break;
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        s_ok = ok;
-    }
-    __syncthreads();
-    return s_ok != 0;
-}
-
-// One single-block scan-free top-down level L: expands list cb of S (n
-// entries, every degree >= 1, degree sum at most XB_SOLO_MF) into list cb ^ 1.
-//  - exclusive prefix of the degrees (XB_SOLO_EPT entries per thread, block
-//    scan);
-//  - each thread takes a contiguous range of at most XB_SOLO_K positions of
-//    the concatenated rows: one binary search for its first entry, then a
-//    walk that moves to the next entry when the position passes its row (an
-//    entry has at least one edge, so one step at a time), issuing every
-//    neighbour load before the first is used;
-//  - each neighbour is claimed by an atomic OR of its visited bit (the
-//    returned word tells whether this lane set it); a winner stores its level,
-//    parent and bit of F_{L+1} and appends itself to the next list (one LDS
-//    atomic per wave and position);
-//  - after a block barrier the next list's row bounds are loaded.
-// Outputs (LDS): *s_nn = winners (at most the level's degree sum, so the
-// list holds them all), *s_mm = their degree sum, *s_zero != 0 when a winner
-// has no edges.  Every thread of the block calls it; it ends with a block
-// barrier.
-#define XB_SOLO_EPT (XB_SOLO_NF / XB_PERSIST_BLOCK)
-__device__ __forceinline__ void xb_solo_level(const XbArgs &a, XbSolo &S, int cb, int32_t n,
-                                              int32_t L, int *s_nn, unsigned long long *s_mm,
-                                              int *s_zero)
-{
-    const int tid = (int)threadIdx.x, ln = xb_lane();
-    const int nx = cb ^ 1;
-    int32_t dl[XB_SOLO_EPT];
-    long long loc = 0;
-#pragma unroll
-    for (int k = 0; k < XB_SOLO_EPT; k++) {
-        const int i = XB_SOLO_EPT * tid + k;
-        dl[k] = i < n ? S.d[cb][i] : 0;
-        loc += dl[k];
-    }
-    long long tot = 0;
-    long long ex = xb_block_excl_scan_i64(loc, &tot);
-#pragma unroll
-    for (int k = 0; k < XB_SOLO_EPT; k++) {
-        const int i = XB_SOLO_EPT * tid + k;
-        if (i < n) S.pf[i] = (int32_t)ex;
-        ex += dl[k];
-    }
-    if (tid == 0) {
-        S.pf[n] = (int32_t)tot;
-        *s_nn = 0;
-        *s_mm = 0ull;
-        *s_zero = 0;
-    }
-    __syncthreads();
-    const int32_t T = (int32_t)tot;
-    const int32_t K = (T + (int32_t)blockDim.x - 1) / (int32_t)blockDim.x; // <= XB_SOLO_K
-    const int32_t p0 = tid * K;
-    const int32_t pe = min(p0 + K, T);
-    int lo = 0;
-    if (p0 < T) {
-        int hi = n - 1; // largest i with pf[i] <= p0
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (S.pf[mid] <= p0) lo = mid;
-            else hi = mid - 1;
-        }
-    }
-    int32_t w[XB_SOLO_K], u[XB_SOLO_K];
-    {
-        int i = lo;
-        int32_t nxt = S.pf[i + 1];
-        int32_t cu = S.v[cb][i];
-        long long rb = S.b[cb][i] - S.pf[i];
-#pragma unroll
-        for (int j = 0; j < XB_SOLO_K; j++) {
-            const int32_t p = p0 + j;
-            w[j] = -1;
-            u[j] = 0;
-            if (p < pe) {
-                if (p >= nxt) {
-                    const int32_t st = nxt;
-                    i++;
-                    nxt = S.pf[i + 1];
-                    cu = S.v[cb][i];
-                    rb = S.b[cb][i] - st;
-                }
-                u[j] = cu;
-                w[j] = a.adj[rb + p];
-            }
-        }
-    }
-    uint32_t *vis32 = reinterpret_cast<uint32_t *>(a.vis);
-    uint32_t *F1 = reinterpret_cast<uint32_t *>(xb_f(a, L + 1));
-    uint32_t old[XB_SOLO_K];
-#pragma unroll
-    for (int j = 0; j < XB_SOLO_K; j++)
-        old[j] = w[j] >= 0 ? atomicOr(&vis32[w[j] >> 5], 1u << (w[j] & 31)) : 0xffffffffu;
-    const int32_t L1 = L + 1;
-#pragma unroll
-    for (int j = 0; j < XB_SOLO_K; j++) {
-        const int32_t v = w[j];
-        const bool won = v >= 0 && !((old[j] >> (v & 31)) & 1u);
-        if (won) {
-            __builtin_nontemporal_store(L1, &a.level[v]);
-            if (a.parent) a.parent[v] = u[j];
-            __hip_atomic_fetch_or(&F1[v >> 5], 1u << (v & 31), __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
-        }
-        const unsigned long long m = __ballot(won);
-        if (m) {
-            const int leader = __ffsll(m) - 1;
-            int base = 0;
-            if (ln == leader) base = atomicAdd(s_nn, (int)__popcll(m));
-            base = __shfl(base, leader);
-            const int idx = base + xb_rank(m);
-            if (won && idx < XB_SOLO_NF) S.v[nx][idx] = v;
-        }
-    }
-    __syncthreads();
-    const int nn = min(*s_nn, XB_SOLO_NF);
-    unsigned long long mm = 0;
-    int zero = 0;
-    long long bb[XB_SOLO_EPT], ee[XB_SOLO_EPT];
-#pragma unroll
-    for (int k = 0; k < XB_SOLO_EPT; k++) {
-        const int i = tid + k * (int)blockDim.x;
-        bb[k] = ee[k] = 0;
-        if (i < nn) {
-            const int32_t v = S.v[nx][i];
-            bb[k] = a.beg[v];
-            ee[k] = a.beg[v + 1];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < XB_SOLO_EPT; k++) {
-        const int i = tid + k * (int)blockDim.x;
-        if (i < nn) {
-            const long long dv = ee[k] - bb[k];
-            S.b[nx][i] = bb[k];
-            S.d[nx][i] = (int32_t)dv;
-            mm += (unsigned long long)dv;
-            zero |= dv == 0;
-        }
-    }
-    mm = xb_wave_sum_u64(mm);
-    if (ln == 0 && mm) atomicAdd(s_mm, mm);
-    if (__ballot(zero != 0) && ln == 0) *s_zero = 1;
-    __syncthreads();
-}
-
-// Block-uniform state of the single-block levels (LDS of block 0).
-struct XbSoloState {
-    int32_t in;      // block 0 runs the levels alone
-    int32_t cb;      // list of S holding the current level
-    int32_t n;       // its entries
-    int32_t nprev;   // entries of the other list (the previous level)
-    int32_t partial; // the first list lacks frontier vertices of degree 0
-    int32_t zero;    // the last level discovered a vertex of degree 0
-    int32_t nn;      // vertices the last level discovered
-    int32_t pad;
-    unsigned long long mm; // their degree sum
-    int32_t q[XB_BINS + 1];
-};
-
-// Entry into single-block levels at level ls.st.L, by block 0 (every thread)
-// while every other block parks at the grid barrier: waits for their
-// arrivals, loads the level's binned queue (exact) into list 0 of S with the
-// row bounds, and clears the (count, degree sum) of level L that the level
-// before left in its ring slot (single-block levels keep their statistics in
-// LDS and write no column of `part`, so every slot then holds what the
-// zeroing schedule would leave).  A queue that lacks frontier vertices of
-// degree 0 (never queued) marks the list partial: its level runs, and the
-// levels return to the grid at the next decision.  Should the queue not fit
-// the list (it always does: it has at most n_f <= XB_SOLO_NF entries), block
-// 0 releases the parked blocks and the level runs on the grid.  Returns
-// false on a barrier error.
-__device__ __forceinline__ bool xb_solo_enter(const XbArgs &a, XbLoopShared &ls, XbSolo &S,
-                                              XbSoloState &ss, int nb)
-{
-    XbCtl *c = a.ctl;
-    const int tid = (int)threadIdx.x;
-    if (!xb_solo_wait_others(c, (uint32_t)(nb - 1))) return false;
-    const int32_t L = ls.st.L;
-    const int qb = L % XB_QBUFS;
-    if (tid < XB_BINS) ss.q[tid + 1] = xb_ld_i32(&c->qcnt[qb][tid].v);
-    __syncthreads();
-    if (tid == 0) {
-        ss.q[0] = 0;
-        for (int k = 0; k < XB_BINS; k++) ss.q[k + 1] += ss.q[k];
-    }
-    __syncthreads();
-    const int32_t n0 = ss.q[XB_BINS];
-    if (n0 <= 0 || n0 > XB_SOLO_NF) {
-        if (tid == 0) {
-            ls.solo = 0;
-            ls.small = 0;
-            *a.lsg = ls;
-        }
-        return xb_grid_sync(c);
-    }
-    for (int i = tid; i < n0; i += (int)blockDim.x) {
-        int k = 0;
-        while (i >= ss.q[k + 1]) k++;
-        const int32_t v = xb_q(a, qb, k)[i - ss.q[k]];
-        const long long b = a.beg[v];
-        S.v[0][i] = v;
-        S.b[0][i] = b;
-        S.d[0][i] = (int32_t)(a.beg[v + 1] - b);
-    }
-    for (int i = tid; i < 2 * XB_PMAX; i += (int)blockDim.x)
-        a.part[(int64_t)xb_f_ring(L, i / XB_PMAX) * XB_PMAX + (i % XB_PMAX)] = 0;
-    if (tid == 0) {
-        ss.in = 1;
-        ss.cb = 0;
-        ss.n = n0;
-        ss.nprev = 0;
-        ss.partial = n0 != ls.n_f;
-        ss.zero = 0;
-    }
-    __syncthreads();
-    return true;
-}
-
-// Leaving single-block levels at level L = ls.st.L (block 0, every thread):
-// the current list becomes the binned queue of level L (buffer L%3), without
-// its degree-0 entries, as the scan-free phase would have queued it, so the
-// queue stays exact; the counters of the other two buffers (levels L+1 and
-// L+2, nothing queued yet) are zeroed.  One LDS cursor per bin; order within
-// a bin is free.
-__device__ __forceinline__ void xb_solo_requeue(const XbArgs &a, const XbLoopShared &ls,
-                                                const XbSolo &S, const XbSoloState &ss)
-{
-    __shared__ int s_bc[XB_BINS];
-    const int tid = (int)threadIdx.x, ln = xb_lane();
-    const int qb = ls.st.L % XB_QBUFS;
-    if (tid < XB_BINS) s_bc[tid] = 0;
-    __syncthreads();
-    const int n = ss.n, cb = ss.cb;
-    for (int i0 = 0; i0 < n; i0 += (int)blockDim.x) {
-        const int i = i0 + tid;
-        int bin = -1;
-        int32_t v = 0;
-        if (i < n) {
-            v = S.v[cb][i];
-            const int32_t d = S.d[cb][i];
-            if (d > 0) bin = xb_bin_of(a, d);
-        }
-#pragma unroll
-        for (int k = 0; k < XB_BINS; k++) {
-            const unsigned long long m = __ballot(bin == k);
-            if (m) {
-                const int leader = __ffsll(m) - 1;
-                int base = 0;
-                if (ln == leader) base = atomicAdd(&s_bc[k], (int)__popcll(m));
-                base = __shfl(base, leader);
-                if (bin == k) xb_q(a, qb, k)[base + xb_rank(m)] = v;
-            }
-        }
-    }
-    __syncthreads();
-    if (tid < XB_QBUFS * XB_BINS) {
-        const int b = tid / XB_BINS, k = tid % XB_BINS;
-        xb_st(&a.ctl->qcnt[b][k].v, b == qb ? s_bc[k] : 0);
-    }
-}
-
 // The level loop's block-uniform state lives in LDS (thread 0 updates it,
 // everyone reads it after a barrier) so it does not hold scalar registers
 // across the phases, which are register-bound.
@@ -1371,17 +1022,12 @@ __device__ __forceinline__ void xb_solo_requeue(const XbArgs &a, const XbLoopSha
 __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
 {
     const int bid = blockIdx.x, nb = gridDim.x;
+    __shared__ XbSmem sm;
     __shared__ XbLoopShared ls;
     __shared__ float s_mlp[XB_MLP_NPARAM]; // learned policy weights, loaded once (transposed)
-    __shared__ float s_mx[XB_MLP_NI], s_my[XB_MLP_NO]; // MLP inputs and outputs
     __shared__ long long s_carry[2];       // see ph_td / ph_bu
-    __shared__ XbSoloState ss;             // single-block levels (block 0)
-    // dynamic LDS (XB_LDS_PERSIST bytes): the phases' XbSmem and per-wave
-    // regions, or the lists of the single-block levels
     extern __shared__ __align__(16) unsigned char xb_dyn_lds[];
-    XbSmem &sm = *reinterpret_cast<XbSmem *>(xb_dyn_lds);
-    XbWaveLds *lds = reinterpret_cast<XbWaveLds *>(xb_dyn_lds + XB_LDS_WAVES_OFF);
-    XbSolo &solo = *reinterpret_cast<XbSolo *>(xb_dyn_lds);
+    XbWaveLds *lds = reinterpret_cast<XbWaveLds *>(xb_dyn_lds);
     if (a.mlp)
         for (int i = threadIdx.x; i < XB_MLP_NPARAM; i += blockDim.x) s_mlp[xb_mlp_lds_pos(i)] = a.mlp[i];
     ph_init(a, bid, nb);
@@ -1395,8 +1041,6 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
         s_carry[1] = 0;
         xb_loop_init(&ls.st);
         if (a.nsrc > 0) ls.st.q_valid = 0; // the level-0 queue comes from a sweep of F_0
-        ls.fix_ring = 0;
-        ss.in = 0;
     }
     __syncthreads();
     XbCtl *c = a.ctl;
@@ -1408,7 +1052,6 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
         const int32_t L = ls.st.L;
         const unsigned long long t0 = wall_clock64();
         const bool need_sweep = ls.st.need_sweep != 0;
-        const bool in_solo = ss.in != 0; // block 0 alone on single-block levels
         if (need_sweep) {
             ph_sweep(a, L, bid, nact, true, true, lds, true);
             if (!xb_level_sync(c, nact)) return;
@@ -1417,20 +1060,9 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
         // other waves zero what level L+1 will write; one block barrier
         // publishes the decision.  Blocks that sit out small-frontier levels
         // and rejoin later skip this level's zeroing: the active blocks did it.
-        // On single-block levels the statistics come from LDS and the zeroing
-        // is the words of F_{L-1}, cleared from the previous level's list.
         // Wave 1 of block 0 meanwhile sums the previous level's bottom-up work
         // into its record (zeros after a top-down level).
-        if (in_solo) {
-            if (threadIdx.x >= 64) {
-                uint64_t *F = xb_f(a, L - 1);
-                const int pc = ss.cb ^ 1;
-                for (int i = (int)threadIdx.x - 64; i < ss.nprev; i += (int)blockDim.x - 64)
-                    xb_st(&F[solo.v[pc][i] >> 6], (uint64_t)0);
-            }
-        } else if (threadIdx.x >= 128) {
-            ph_level_prep(a, L, bid, nact, 128);
-        }
+        if (threadIdx.x >= 128) ph_level_prep(a, L, bid, nact, 128);
         if (threadIdx.x >= 64 && threadIdx.x < 128 && bid == 0 && L > 0 && L - 1 < a.rec_cap) {
             long long wv[3] = {0, 0, 0};
             if (ls.st.prev_mode == XBFS_MODE_BU) {
@@ -1445,61 +1077,70 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
             }
         }
         if (threadIdx.x < 64) {
-            long long sv[5] = {0, 0, 0, 0, 0};
-            if (in_solo) {
-                sv[0] = ss.nn;
-                sv[1] = (long long)ss.mm;
-            } else {
-                const int ff[5] = {need_sweep ? xb_f_sweep(L, 0) : xb_f_ring(L, 0),
-                                   need_sweep ? xb_f_sweep(L, 1) : xb_f_ring(L, 1),
-                                   xb_f_ring(L + 1, 0), xb_f_ring(L + 1, 1), xb_f_ring(L + 1, 2)};
-                xb_part_sum<5>(a, ff, sv, nact);
-            }
-            xb_decide(a, ls, s_mlp, s_mx, s_my, L, sv, bid);
+            long long sv[5];
+            const int ff[5] = {need_sweep ? xb_f_sweep(L, 0) : xb_f_ring(L, 0),
+                               need_sweep ? xb_f_sweep(L, 1) : xb_f_ring(L, 1),
+                               xb_f_ring(L + 1, 0), xb_f_ring(L + 1, 1), xb_f_ring(L + 1, 2)};
+            xb_part_sum<5>(a, ff, sv, nact);
             if (threadIdx.x == 0) {
-                // the level's queue is exact: scan-free after scan-free, no
-                // discoveries two levels ahead (a single-block list is exact
-                // unless it began partial or holds a vertex of degree 0)
-                const bool exact = ls.open && ls.mode == XBFS_MODE_TD_SF && ls.st.q_valid &&
-                                   !need_sweep && ls.p2_next == 0 &&
-                                   !(in_solo && (ss.partial || ss.zero));
-                ls.solo = a.solo && exact && ls.n_f <= XB_SOLO_NF && ls.m_f <= XB_SOLO_MF;
-                ls.small = !ls.solo && small_ok && exact && ls.n_f <= XB_SMALL_NF &&
-                           ls.m_f <= XB_SMALL_MF;
+                long long n_u = 0, m_u = 0;
+                const bool open = xb_level_open(ls.st, a, sv[0], sv[1], sv[2], sv[3], &n_u, &m_u);
+                ls.open = open ? 1 : 0;
+                ls.n_f = sv[0];
+                ls.m_f = sv[1];
+                ls.n_u = n_u;
+                ls.m_u = m_u;
+                ls.p2_next = sv[4];
+                if (!open || L >= a.max_levels) {
+                    if (bid == 0) {
+                        const long long pend_n = sv[2], pend_m = sv[3];
+                        xb_st(&c->n_levels, L);
+                        xb_st(&c->max_level, pend_n > 0 ? L + 1 : (sv[0] > 0 ? L : L - 1));
+                        xb_st(&c->v_cc, (long long)(ls.st.cum_n + pend_n));
+                        xb_st(&c->m_cc, (long long)(ls.st.cum_m + pend_m));
+                        if (open) xb_st(&c->error, 2);
+                    }
+                    ls.open = 0;
+                }
+            }
+            xb_wave_sync();
+            if (ls.open) {
+                const float *pred = nullptr;
+                if (a.mlp && L <= XBFS_MLP_MAX_LEVEL) {
+                    if (threadIdx.x == 0)
+                        xbfs_mlp_features(L, ls.n_f, ls.m_f, ls.n_u, ls.m_u, ls.st.prev_mode,
+                                          sm.mlp.x);
+                    xb_wave_sync();
+                    xb_mlp_wave_forward(s_mlp, sm.mlp.x, sm.mlp.y);
+                    xb_wave_sync();
+                    pred = sm.mlp.y;
+                }
+                if (threadIdx.x == 0) {
+                    int32_t htag = 0, used = 0;
+                    ls.mode = xbfs_choose_mode(a.policy, a.sched, a.sched_len, pred, &ls.st.hs, L,
+                                               ls.m_f, a.m, a.alpha, &htag, &used);
+                    ls.htag = htag;
+                    ls.used = used;
+                    ls.small = small_ok && ls.mode == XBFS_MODE_TD_SF && ls.st.q_valid &&
+                               !need_sweep && ls.p2_next == 0 && ls.n_f <= XB_SMALL_NF &&
+                               ls.m_f <= XB_SMALL_MF;
+                }
             }
         }
         __syncthreads();
-        // Blocks taking part in the level: the grid, the first
-        // XB_SMALL_BLOCKS on small-frontier levels, block 0 alone on
-        // single-block levels (every block takes the same branch: the inputs
-        // are identical).  Blocks that sit out park at the grid barrier and
-        // continue from the loop state the active blocks publish when they
-        // arrive there too.
-        const int want = !ls.open ? nb : ls.solo ? 1 : ls.small ? XB_SMALL_BLOCKS : nb;
-        if (in_solo && !(ls.open && ls.solo)) {
-            // leaving single-block levels (block 0; with a one-block grid
-            // too, where want is always 1): the list becomes the level's
-            // queue, the level runs on the whole grid, the parked blocks
-            // rejoin
-            if (ls.open) xb_solo_requeue(a, ls, solo, ss);
-            if (threadIdx.x == 0) {
-                if (ls.open) {
-                    ls.small = 0;
-                    ls.fix_ring = ss.partial;
-                }
-                ss.in = 0;
-                *a.lsg = ls;
+        // entering / leaving small-frontier levels (every active block takes
+        // the same branch: the inputs are identical)
+        if (!ls.open || (nact < nb && !ls.small)) {
+            if (nact < nb) {
+                // the parked blocks rejoin: they get the loop state, decision
+                // included, from device memory after the grid barrier
+                if (bid == 0 && threadIdx.x == 0) *a.lsg = ls;
+                if (!xb_grid_sync(c)) return;
+                nact = nb;
             }
-            if (!xb_grid_sync(c)) return;
-            nact = nb;
-        } else if (want > nact) {
-            // the parked blocks rejoin: they get the loop state, decision
-            // included, from device memory after the grid barrier
-            if (bid == 0 && threadIdx.x == 0) *a.lsg = ls;
-            if (!xb_grid_sync(c)) return;
-            nact = nb;
-        } else if (want < nact) {
-            if (bid >= want) {
+            if (!ls.open) return;
+        } else if (nact == nb && ls.small) {
+            if (bid >= XB_SMALL_BLOCKS) {
                 if (!xb_park(c)) return;
                 if (threadIdx.x == 0) {
                     ls = *a.lsg;
@@ -1507,42 +1148,23 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
                     s_carry[1] = 0;
                 }
                 __syncthreads();
-                nact = nb;
+                if (!ls.open) return;
             } else {
-                nact = want;
+                nact = XB_SMALL_BLOCKS;
             }
-        }
-        if (!ls.open) return;
-        if (ls.solo && !ss.in) {
-            // block 0 alone (every other block parked above)
-            if (!xb_solo_enter(a, ls, solo, ss, nb)) return;
-            if (!ss.in) nact = nb; // not entered: the parked blocks were released
         }
         // the level itself, on the active blocks; a parked block that just
         // rejoined takes the level from the copied state
         const int32_t Lc = ls.st.L;
         const int32_t mode = ls.mode;
-        if (ss.in) {
-            xb_solo_level(a, solo, ss.cb, ss.n, Lc, &ss.nn, &ss.mm, &ss.zero);
-        } else {
-            if (ls.fix_ring) {
-                // after single-block levels whose first list lacked degree-0
-                // frontier vertices: bitmap ring (Lc+3)&3, which no phase of
-                // this level touches, cleared whole (every block: nact == nb)
-                uint64_t *F3 = xb_f(a, Lc + 3);
-                for (int64_t w = (int64_t)bid * blockDim.x + threadIdx.x; w < a.nwords;
-                     w += (int64_t)nb * blockDim.x)
-                    F3[w] = 0ull;
-            }
-            if (mode != XBFS_MODE_BU && !ls.st.q_valid && !ls.st.need_sweep) {
-                ph_sweep(a, Lc, bid, nact, false, false, lds);
-                if (!xb_level_sync(c, nact)) return;
-            }
-            if (mode == XBFS_MODE_TD_SF) ph_td<true>(a, Lc, bid, nact, ls.p2_next == 0, sm, s_carry);
-            else if (mode == XBFS_MODE_TD_SS) ph_td<false>(a, Lc, bid, nact, false, sm, s_carry);
-            else ph_bu(a, Lc, bid, nact, lds, s_carry);
+        if (mode != XBFS_MODE_BU && !ls.st.q_valid && !ls.st.need_sweep) {
+            ph_sweep(a, Lc, bid, nact, false, false, lds);
             if (!xb_level_sync(c, nact)) return;
         }
+        if (mode == XBFS_MODE_TD_SF) ph_td<true>(a, Lc, bid, nact, ls.p2_next == 0, sm, s_carry);
+        else if (mode == XBFS_MODE_TD_SS) ph_td<false>(a, Lc, bid, nact, false, sm, s_carry);
+        else ph_bu(a, Lc, bid, nact, lds, s_carry);
+        if (!xb_level_sync(c, nact)) return;
         if (threadIdx.x == 0) {
             if (bid == 0) {
                 if (nact < nb) xb_st(&c->heartbeat, (uint32_t)Lc);
@@ -1564,12 +1186,6 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
             ls.st.need_sweep = (mode == XBFS_MODE_TD_SS);
             ls.st.prev_mode = mode;
             ls.st.L = Lc + 1;
-            ls.fix_ring = 0;
-            if (ss.in) {
-                ss.nprev = ss.n;
-                ss.n = ss.nn;
-                ss.cb ^= 1;
-            }
         }
         __syncthreads();
     }
@@ -1584,10 +1200,9 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_coop_level(XbArgs a, int32
                                                                  int32_t q_valid, int32_t need_sweep,
                                                                  int32_t enqueue, int32_t extra_sweep)
 {
-    // dynamic LDS (XB_LDS_PHASES bytes), laid out as in k_persistent
+    __shared__ XbSmem sm;
     extern __shared__ __align__(16) unsigned char xb_dyn_lds[];
-    XbSmem &sm = *reinterpret_cast<XbSmem *>(xb_dyn_lds);
-    XbWaveLds *lds = reinterpret_cast<XbWaveLds *>(xb_dyn_lds + XB_LDS_WAVES_OFF);
+    XbWaveLds *lds = reinterpret_cast<XbWaveLds *>(xb_dyn_lds);
     const int bid = blockIdx.x, nb = gridDim.x;
     XbCtl *c = a.ctl;
     ph_level_prep(a, L, bid, nb);

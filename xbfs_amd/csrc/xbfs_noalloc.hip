@@ -191,7 +191,6 @@ extern "C" int xbfs_bfs_noalloc(int device, const int64_t *d_beg, const int32_t 
     a.alpha = o.alpha;
     a.max_levels = (int32_t)std::min<int64_t>(n + 1, INT32_MAX - 4);
     if (o.max_levels > 0) a.max_levels = std::min(a.max_levels, o.max_levels);
-    a.solo = o.single_block == 0 ? 1 : 0;
 
     int grid = dv.coop_blocks;
     if (o.grid_dim > 0 && o.grid_dim < grid) grid = o.grid_dim;
