@@ -129,6 +129,14 @@ int xbfs_create(int device, const int64_t *beg_pos, const int32_t *adj,
                 int64_t n, int64_t m, xbfs_ctx **out);
 void xbfs_destroy(xbfs_ctx *ctx);
 int64_t xbfs_num_vertices(const xbfs_ctx *ctx);
+/* Persistent traversals are launched as plain kernels with a residency check
+ * (a plain launch costs about 6 us, a cooperative one about 19 us); when not
+ * every block was resident at once, the traversal is relaunched
+ * cooperatively and the context launches cooperatively from then on.  Returns
+ * how many traversals were relaunched (XBFS_LAUNCH=coop: every launch is
+ * cooperative; XBFS_RESIDENCY_US bounds the check, default 20000).  No
+ * reference counterpart. */
+int64_t xbfs_launch_fallbacks(const xbfs_ctx *ctx);
 int64_t xbfs_num_edges(const xbfs_ctx *ctx);
 
 /* direction / queue modes (the reference's per-level tag in parentheses) */

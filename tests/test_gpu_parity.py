@@ -148,6 +148,31 @@ def test_barrier_wait_limit_then_full_run():
     ctx.close()
 
 
+def test_launch_kinds(monkeypatch):
+    """The persistent kernel is launched plainly with a residency check at its
+    first barrier, cooperatively under XBFS_LAUNCH=coop, and with a check
+    bound of one clock tick (XBFS_RESIDENCY_US=0) the first traversal's
+    blocks cannot all arrive in time: it is relaunched cooperatively, and the
+    context stays cooperative.  Levels and parents equal the oracle's in every
+    case."""
+    g = Graph.rmat(12, 16, seed=1012)
+    srcs = seeded_sources(g, 3)
+    for env, want_fb in ((None, 0), ("coop", 0), ("tick", 1)):
+        monkeypatch.delenv("XBFS_LAUNCH", raising=False)
+        monkeypatch.delenv("XBFS_RESIDENCY_US", raising=False)
+        if env == "coop":
+            monkeypatch.setenv("XBFS_LAUNCH", "coop")
+        elif env == "tick":
+            monkeypatch.setenv("XBFS_RESIDENCY_US", "0")
+        ctx = Context(g)
+        for s in srcs:
+            for pol in POLICIES[:4]:
+                say("launch", env, s, pol)
+                _check_one(ctx, g, s, persistent=1, **pol)
+        assert ctx.launch_fallbacks() == want_fb, (env, ctx.launch_fallbacks())
+        ctx.close()
+
+
 def test_hub_component_termination(monkeypatch):
     """A traversal from the largest-degree vertex's component ends as soon as
     every member has its level, one level (or, after +2 discoveries, two)

@@ -96,6 +96,7 @@ SIGNATURES = {
     "xbfs_destroy": (None, [c_void_p]),
     "xbfs_num_vertices": (c_int64, [c_void_p]),
     "xbfs_num_edges": (c_int64, [c_void_p]),
+    "xbfs_launch_fallbacks": (c_int64, [c_void_p]),
     "xbfs_opts_default": (None, [POINTER(Opts)]),
     "xbfs_bfs": (c_int, [c_void_p, c_int32, POINTER(Opts), c_void_p, c_void_p,
                          POINTER(LevelRec), c_int32, POINTER(Stats)]),
@@ -293,6 +294,11 @@ class Context:
         self.h = h
         self.n, self.m = graph.n, graph.m
         _live.add(self)
+
+    def launch_fallbacks(self):
+        """Persistent traversals relaunched cooperatively after a plain
+        launch's residency check failed (xbfs_launch_fallbacks)."""
+        return int(lib().xbfs_launch_fallbacks(self.h))
 
     def close(self):
         if self.h:

@@ -78,6 +78,13 @@ extern "C" {
 
 // Bound of one grid-barrier wait in the persistent kernel, in wall-clock
 // ticks: opts.wait_limit_us, or 4 s.
+// Bound of the first barrier's wait under a plain launch (see bfs_persistent).
+static unsigned long long res_ticks(const xbfs_ctx *c)
+{
+    const double t = c->res_us * 1e-6 * c->wall_hz;
+    return t < 1.0 ? 1ull : (unsigned long long)t;
+}
+
 static unsigned long long wait_ticks(const xbfs_ctx *c, const xbfs_opts &o)
 {
     const double us = o.wait_limit_us > 0 ? (double)o.wait_limit_us : 4.0e6;
@@ -108,6 +115,7 @@ void xbfs_opts_default(xbfs_opts *o)
 }
 
 int64_t xbfs_num_vertices(const xbfs_ctx *c) { return c ? c->n : -1; }
+int64_t xbfs_launch_fallbacks(const xbfs_ctx *c) { return c ? c->plain_fallbacks : -1; }
 int64_t xbfs_num_edges(const xbfs_ctx *c) { return c ? c->m : -1; }
 
 // Builds the bottom-up copy of the adjacency, c->d_adj_bu: every row ordered
@@ -202,6 +210,8 @@ int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, i
         hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess)
         return fail(XBFS_ERR_HIP);
     if (clk_khz > 0) c->wall_hz = (double)clk_khz * 1e3;
+    if (const char *e = getenv("XBFS_LAUNCH")) c->launch_plain = strcmp(e, "coop") != 0;
+    if (const char *e = getenv("XBFS_RESIDENCY_US")) c->res_us = atof(e);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent, XB_PERSIST_BLOCK,
                                                      XB_LDS_PERSIST) !=
@@ -506,22 +516,49 @@ static int bfs_host_loop(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs_
     return XBFS_OK;
 }
 
-// Persistent driver: the whole level loop in one cooperative launch.
+// Persistent driver: the whole level loop in one launch of at most one block
+// per CU.  A plain launch costs about 6 us on MI355X, a cooperative one about
+// 19 us (an empty kernel of the same shape, event to event), so the kernel is
+// launched plainly while the GPU shows it every block at once: its first
+// barrier then bounds its wait by res_ticks (XBFS_RESIDENCY_US, default
+// 20 ms) and, when some block did not arrive in time, every block leaves with
+// code 3; the traversal is then relaunched cooperatively (co-residency
+// guaranteed), and the context keeps launching cooperatively from then on.
+// XBFS_LAUNCH=coop makes every launch cooperative.  The device time of a
+// relaunched traversal includes the first attempt.
 static int bfs_persistent(xbfs_ctx *c, const XbArgs &a, const xbfs_opts &o, xbfs_level_rec *rec,
                           int32_t rec_cap, xbfs_stats *st)
 {
     int grid = c->coop_blocks;
     if (o.grid_dim > 0 && o.grid_dim < grid) grid = o.grid_dim;
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl, wait_ticks(c, o));
+    const unsigned long long res = c->launch_plain ? res_ticks(c) : 0ull;
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl, wait_ticks(c, o), res);
     XB_LAUNCHED();
     XB_HIP(hipEventRecord(c->ev_start, c->stream));
     XbArgs args = a;
     void *kargs[] = {&args};
-    XB_HIP(hipLaunchCooperativeKernel((const void *)k_persistent, dim3(grid), dim3(XB_PERSIST_BLOCK),
-                                      kargs, XB_LDS_PERSIST, c->stream));
+    if (res)
+        XB_HIP(hipLaunchKernel((const void *)k_persistent, dim3(grid), dim3(XB_PERSIST_BLOCK), kargs,
+                               XB_LDS_PERSIST, c->stream));
+    else
+        XB_HIP(hipLaunchCooperativeKernel((const void *)k_persistent, dim3(grid),
+                                          dim3(XB_PERSIST_BLOCK), kargs, XB_LDS_PERSIST, c->stream));
     XB_HIP(hipEventRecord(c->ev_stop, c->stream));
     XB_HIP(hipEventSynchronize(c->ev_stop));
     XB_HIP(hipMemcpy(c->h_ctl, c->d_ctl, sizeof(XbCtl), hipMemcpyDeviceToHost));
+    if (res && c->h_ctl->error == 3) {
+        // some block was not resident with the others: cooperative from now on
+        c->launch_plain = 0;
+        c->plain_fallbacks++;
+        hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl, wait_ticks(c, o),
+                           0ull);
+        XB_LAUNCHED();
+        XB_HIP(hipLaunchCooperativeKernel((const void *)k_persistent, dim3(grid),
+                                          dim3(XB_PERSIST_BLOCK), kargs, XB_LDS_PERSIST, c->stream));
+        XB_HIP(hipEventRecord(c->ev_stop, c->stream));
+        XB_HIP(hipEventSynchronize(c->ev_stop));
+        XB_HIP(hipMemcpy(c->h_ctl, c->d_ctl, sizeof(XbCtl), hipMemcpyDeviceToHost));
+    }
     const XbCtl &h = *c->h_ctl;
     if (h.error == 1) return XBFS_ERR_DEVICE;
     if (h.error == 2) return XBFS_ERR_LIMIT;
@@ -967,7 +1004,7 @@ int xbfs_bfs_batch64(xbfs_ctx *c, const int32_t *srcs, int32_t nsrc, const xbfs_
     if (o.policy == XBFS_POLICY_SCHEDULE)
         XB_HIP(hipMemcpyAsync(c->d_sched, o.schedule, (size_t)o.schedule_len,
                               hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl, wait_ticks(c, o));
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl, wait_ticks(c, o), 0ull);
     XB_LAUNCHED();
     XB_HIP(hipEventRecord(c->ev_b64[0], c->stream));
     void *kargs[] = {&a};
@@ -1166,7 +1203,7 @@ extern "C" int xbfs_bfs_probe(xbfs_ctx *c, int32_t src, const xbfs_opts *opts, i
                               hipMemcpyHostToDevice, c->stream));
     const XbArgs a = make_args(c, src, o);
     const int grid = o.grid_dim > 0 ? o.grid_dim : c->num_cu * 8;
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl, wait_ticks(c, o));
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, c->d_ctl, wait_ticks(c, o), 0ull);
     hipLaunchKernelGGL(k_ph_init, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a);
     if (a.nsrc > 0) hipLaunchKernelGGL(k_ph_seed, dim3(grid), dim3(XB_PBLOCK), 0, c->stream, a);
     XB_LAUNCHED();

@@ -37,6 +37,9 @@ struct xbfs_ctx {
     int2 *d_hd = nullptr;       // {largest-degree neighbour, degree} per vertex
     int32_t *d_adj_bu = nullptr; // bottom-up copy of d_adj, rows by neighbour degree (or NULL)
     int32_t hub_first = 0;      // d_adj_bu is built (each row starts with its hub)
+    int launch_plain = 1;       // persistent kernel: plain launch with a residency check
+    double res_us = 20000.0;    // bound of that check (first barrier wait), microseconds
+    int64_t plain_fallbacks = 0; // plain launches that were relaunched cooperatively
     uint64_t *d_vis = nullptr;  // visited bitmap
     uint64_t *d_fbm = nullptr;  // 4 frontier bitmaps (ring by level & 3)
     int32_t *d_level = nullptr; // int32[n]

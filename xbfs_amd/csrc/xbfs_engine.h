@@ -43,10 +43,11 @@ struct alignas(256) XbCtl {
     uint32_t sbar_pad1[31];
     uint32_t heartbeat;                 // bumped once per small-frontier level
     uint32_t hb_pad[31];
-    int32_t error;                      // 1 barrier timeout, 2 level cap
+    int32_t error;                      // 1 barrier timeout, 2 level cap, 3 blocks not co-resident
     int32_t err_pad0;
     unsigned long long wait_ticks;      // bound of a barrier wait (wall-clock ticks), per launch
-    int32_t err_pad[28];
+    unsigned long long res_ticks;       // plain launch: bound of the first barrier's wait, 0 otherwise
+    int32_t err_pad[26];
     XbCounter qcnt[XB_QBUFS][XB_BINS];  // queue lengths, one cache line each
     int32_t n_levels, max_level, pad0, pad1;
     long long v_cc, m_cc;

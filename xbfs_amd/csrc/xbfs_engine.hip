@@ -190,6 +190,56 @@ __device__ bool xb_barrier(XbCtl *c, uint32_t *cnt, uint32_t *genw, uint32_t mem
     return s_ok != 0;
 }
 
+// The first grid barrier of a persistent launch.  A cooperative launch
+// guarantees that every block is resident at once; a plain launch of one block
+// per CU gets that on a GPU with no other work, but not by contract.  Under a
+// plain launch (ctl->res_ticks > 0) a block that waits here longer than
+// res_ticks reports code 3 and every block leaves at this barrier or the next,
+// so the kernel ends whichever blocks ran; the host then relaunches
+// cooperatively.  With res_ticks = 0 it is xb_grid_sync.
+__device__ bool xb_first_sync(XbCtl *c)
+{
+    __shared__ int s_ok1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        uint32_t *cnt = &c->bar_count, *genw = &c->bar_gen;
+        const uint32_t gen = __hip_atomic_load(genw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t prev = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int ok = 1;
+        if (prev == gridDim.x - 1) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(genw, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const unsigned long long res =
+                __hip_atomic_load(&c->res_ticks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long limit =
+                res ? res : __hip_atomic_load(&c->wait_ticks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int32_t code = res ? 3 : 1;
+            const unsigned long long t0 = wall_clock64();
+            while (__hip_atomic_load(genw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(1);
+                if (__hip_atomic_load(&c->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    ok = 0;
+                    break;
+                }
+                if (wall_clock64() - t0 > limit) {
+                    int32_t none = 0;
+                    __hip_atomic_compare_exchange_strong(&c->error, &none, code, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_ok1 = ok;
+    }
+    __syncthreads();
+    return s_ok1 != 0;
+}
+
 // Grid-wide barrier (every block of the launch).
 __device__ bool xb_grid_sync(XbCtl *c)
 {
@@ -1031,7 +1081,7 @@ __global__ void __launch_bounds__(XB_PERSIST_BLOCK) k_persistent(XbArgs a)
     if (a.mlp)
         for (int i = threadIdx.x; i < XB_MLP_NPARAM; i += blockDim.x) s_mlp[xb_mlp_lds_pos(i)] = a.mlp[i];
     ph_init(a, bid, nb);
-    if (!xb_grid_sync(a.ctl)) return;
+    if (!xb_first_sync(a.ctl)) return;
     if (a.nsrc > 0) {
         ph_seed(a, bid, nb);
         if (!xb_grid_sync(a.ctl)) return;
@@ -1276,10 +1326,11 @@ __global__ void __launch_bounds__(1024) k_part_reduce(XbArgs a)
     }
 }
 
-__global__ void k_ctl_reset(XbCtl *c, unsigned long long wait_ticks)
+__global__ void k_ctl_reset(XbCtl *c, unsigned long long wait_ticks, unsigned long long res_ticks)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         xb_st(&c->wait_ticks, wait_ticks);
+        xb_st(&c->res_ticks, res_ticks);
         xb_st(&c->bar_count, 0u);
         xb_st(&c->bar_gen, 0u);
         xb_st(&c->sbar_count, 0u);

@@ -197,7 +197,7 @@ extern "C" int xbfs_bfs_noalloc(int device, const int64_t *d_beg, const int32_t 
     const double us = o.wait_limit_us > 0 ? (double)o.wait_limit_us : 4.0e6;
     const double ticks = us * 1e-6 * dv.wall_hz;
     hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s0, ctl,
-                       ticks < 1.0 ? 1ull : (unsigned long long)ticks);
+                       ticks < 1.0 ? 1ull : (unsigned long long)ticks, 0ull);
     XB_LAUNCHED();
     XB_HIP(hipStreamSynchronize(s0));
     void *kargs[] = {&a};
