@@ -74,14 +74,14 @@ HBM_PEAK = 8.0e12
 # moves: factor 2.0; WRITE_SIZE reads all of them: factor 1.0).  kernel_us is
 # the mean launch time of the kernel trace taken with them; the figure is
 # reported only while the measured launch time stays within 10 % of it.
-PMC_TRAFFIC = {"workload": (24, 16, 64, "learned"), "fetch_kib": 409197.8, "fetch_factor": 1.99995,
-               "write_kib": 256612.2, "write_factor": 1.0, "kernel_us": 418.2,
+PMC_TRAFFIC = {"workload": (24, 16, 64, "learned"), "fetch_kib": 410486.8, "fetch_factor": 1.99995,
+               "write_kib": 257608.5, "write_factor": 1.0, "kernel_us": 424.9,
                "profile": "profiles/k_persistent_rmat24_pmc_traffic.md"}
 
 MODE_SF, MODE_SS, MODE_BU = 0, 1, 2
 
 
-def traversal_bytes(recs, n, v_cc, parents=False):
+def traversal_bytes(recs, n, v_cc, parents=False, per_level=None):
     """Bytes one traversal reads and writes as executed, from its level
     records.  W = one bitmap (n/8 bytes, rounded up to 64-bit words).
       initialisation: 4n level words (+4n parents), visited + 4 frontier rings (5W)
@@ -98,12 +98,17 @@ def traversal_bytes(recs, n, v_cc, parents=False):
         probed (8 B hub record + 4 B bitmap word), 16 B offsets per scanned row,
         8 B per scanned adjacency word (word + bitmap probe), 4 B level write
         per discovered vertex, visited and next-frontier words written (2W)
-      parents: 4 B per discovered vertex."""
+      parents: 4 B per discovered vertex.
+    per_level (a list): receives (level, mode, bytes) of each level; the
+    initialisation is the entry of level -1, so the entries sum to the total."""
     W = 8 * ((n + 63) // 64)
     b = 4 * n + 5 * W + (4 * n if parents else 0)
+    if per_level is not None:
+        per_level.append((-1, -1, b))
     prev = -1
     seen = sum(r["n_f"] for r in recs)
     for i, r in enumerate(recs):
+        b0 = b
         mode, nf, mf = r["mode"], r["n_f"], r["m_f"]
         nd = recs[i + 1]["n_f"] if i + 1 < len(recs) else max(v_cc - seen, 0)
         b += W
@@ -120,6 +125,8 @@ def traversal_bytes(recs, n, v_cc, parents=False):
         if parents:
             b += 4 * nd
         prev = mode
+        if per_level is not None:
+            per_level.append((r["level"], mode, b - b0))
     return b
 
 
@@ -342,11 +349,11 @@ def main():
     lvl_bytes = {}
     for i, s in enumerate(srcs):
         _, _, recs, st1 = ctx.bfs(int(s), opts=opts, want_levels=False, rec_cap=256)
-        b_do[i] = traversal_bytes(recs, g.n, st1["v_cc"])
-        for r in recs:
-            key = (r["level"], r["mode"])
-            one = traversal_bytes([r], g.n, 0) - traversal_bytes([], g.n, 0)
-            lvl_bytes[key] = lvl_bytes.get(key, 0) + one
+        split = []
+        b_do[i] = traversal_bytes(recs, g.n, st1["v_cc"], per_level=split)
+        for lv_, md_, by_ in split:
+            key = (lv_, md_)
+            lvl_bytes[key] = lvl_bytes.get(key, 0) + by_
     b_teps = 8 * m_cc + 20 * v_cc + 4 * g.n
     t_mean = float(t_dev[main_pol].mean())
     achieved = float(b_do.mean()) / t_mean
@@ -480,8 +487,14 @@ def main():
                      "algorithmic_bytes_per_launch": round(float(b_do.mean())),
                      "bytes_model": "as executed, per level from the level records "
                                     "(bench.traversal_bytes)",
-                     "bytes_per_level_mode": {f"L{k[0]}_{['td_sf', 'td_ss', 'bu'][k[1]]}":
-                                              round(v / len(srcs)) for k, v in sorted(lvl_bytes.items())},
+                     "bytes_per_level_mode": {("init" if k[0] < 0 else
+                                               f"L{k[0]}_{['td_sf', 'td_ss', 'bu'][k[1]]}"):
+                                              round(v / len(srcs))
+                                              for k, v in sorted(lvl_bytes.items())[:40]},
+                     "bytes_per_level_note": "mean over the sources of each (level, mode)'s bytes "
+                                             "(sources that skip a level or run it in another mode "
+                                             "count zero); the entries sum to "
+                                             "algorithmic_bytes_per_launch (first 40 entries)",
                      "teps_equivalent": {"bytes_per_launch": round(float(b_teps.mean())),
                                          "gbs": round(float(b_teps.mean()) / t_mean / 1e9, 2),
                                          "model": "8*m_cc + 20*V_cc + 4*V (every edge of the "

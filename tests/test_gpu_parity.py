@@ -335,12 +335,19 @@ def test_repeat_is_deterministic(rmat10):
 
 
 def test_device_scan_matches_exclusive_scan(rmat10):
+    """The single-pass look-back scan (4096 elements per tile) against the
+    oracle's: one and several tiles, ragged ends, 2^24 elements (4096 tiles
+    looking back over each other), non-negative and signed values (the
+    status words carry signed 62-bit sums), and sums past 2^31."""
     g, ctx, d = rmat10
     rng = np.random.default_rng(11)
-    for n in (1, 2, 63, 4095, 4096, 4097, 100_003, 3_000_000, (1 << 24) - 3, 1 << 24):
-        say("scan", n)
-        x = rng.integers(0, 1 << 20, n, dtype=np.int32)
-        assert (ctx.exclusive_scan(x) == O.exclusive_scan(x)).all()
+    for n in (1, 2, 63, 4095, 4096, 4097, 8191, 100_003, 3_000_000, (1 << 24) - 3, 1 << 24):
+        for lo in (0, -(1 << 20)):
+            say("scan", n, lo)
+            x = rng.integers(lo, 1 << 20, n, dtype=np.int32)
+            assert (ctx.exclusive_scan(x) == O.exclusive_scan(x)).all()
+    x = np.full(1 << 22, (1 << 31) - 1, np.int32)
+    assert (ctx.exclusive_scan(x) == O.exclusive_scan(x)).all()
 
 
 def test_unvisited_list_is_sorted_unvisited_set():

@@ -50,6 +50,23 @@ bool pow2_block(int b) { return b >= 64 && b <= 1024 && (b & (b - 1)) == 0; }
 
 } // namespace
 
+// Exclusive scan of n int32 into int64 on the device, one launch of
+// k_scan_lookback.  scratch: ceil(n / XB_SCAN_CHUNK) status words, then one
+// word for the tile ticket and the error flag, zeroed here.  *err_word points
+// at the error flag (nonzero: a look-back wait ran out).
+static hipError_t xb_scan_i32(hipStream_t s, const int32_t *in, int64_t n, long long *out,
+                              unsigned long long *scratch, int32_t **err_word)
+{
+    const int64_t T = (n + XB_SCAN_CHUNK - 1) / XB_SCAN_CHUNK;
+    hipError_t e = hipMemsetAsync(scratch, 0, sizeof(unsigned long long) * (size_t)(T + 1), s);
+    if (e != hipSuccess) return e;
+    int32_t *tk = reinterpret_cast<int32_t *>(scratch + T);
+    *err_word = tk + 1;
+    hipLaunchKernelGGL(k_scan_lookback, dim3((unsigned)T), dim3(256), 0, s, in, n, scratch, tk, tk + 1,
+                       out);
+    return hipGetLastError();
+}
+
 int xb_ctx_free_device(xbfs_ctx *c)
 {
     (void)hipSetDevice(c->device);
@@ -839,7 +856,7 @@ static int b64_setup(xbfs_ctx *c)
         hipMalloc(&kin, sizeof(uint32_t) * (size_t)n) != hipSuccess ||
         hipMalloc(&kout, sizeof(uint32_t) * (size_t)n) != hipSuccess ||
         hipMalloc(&ids, sizeof(int32_t) * (size_t)n) != hipSuccess ||
-        hipMalloc(&spart, sizeof(long long) * (size_t)T) != hipSuccess ||
+        hipMalloc(&spart, sizeof(long long) * (size_t)(T + 1)) != hipSuccess ||
         hipMalloc(&soff, sizeof(long long) * (size_t)n) != hipSuccess) {
         cleanup();
         return XBFS_ERR_NOMEM;
@@ -862,25 +879,35 @@ static int b64_setup(xbfs_ctx *c)
         hipLaunchKernelGGL(k_b64_rank, dim3(grid), dim3(256), 0, c->stream, c->d_b64_ord, kout, n,
                            c->d_b64_rank, c->d_b64_rdeg);
         // row offsets: exclusive scan of the degrees by rank
-        hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)T), dim3(256), 0, c->stream, c->d_b64_rdeg, n,
-                           spart);
-        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(1024), 0, c->stream, spart, T);
-        hipLaunchKernelGGL(k_scan_down, dim3((unsigned)T), dim3(256), 0, c->stream, c->d_b64_rdeg, n,
-                           spart, reinterpret_cast<long long *>(c->d_b64_rbeg));
+        int32_t *d_err = nullptr;
+        if (xb_scan_i32(c->stream, c->d_b64_rdeg, n, reinterpret_cast<long long *>(c->d_b64_rbeg),
+                        reinterpret_cast<unsigned long long *>(spart), &d_err) != hipSuccess)
+            rc = XBFS_ERR_HIP;
         hipLaunchKernelGGL(k_b64_set_end, dim3(1), dim3(64), 0, c->stream, c->d_b64_rbeg, n, m);
-        if (hipGetLastError() != hipSuccess) rc = XBFS_ERR_HIP;
+        int32_t h_err = 0;
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(&h_err, d_err, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = XBFS_ERR_HIP;
+        else if (h_err)
+            rc = XBFS_ERR_DEVICE;
     }
     // 32-edge segments: count per rank (into the ids buffer, no longer
     // needed), exclusive scan, fill
     if (rc == XBFS_OK) {
         int32_t *scnt = ids;
         hipLaunchKernelGGL(k_b64_seg_count, dim3(grid), dim3(256), 0, c->stream, c->d_b64_rdeg, n, scnt);
-        hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)T), dim3(256), 0, c->stream, scnt, n, spart);
-        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(1024), 0, c->stream, spart, T);
-        hipLaunchKernelGGL(k_scan_down, dim3((unsigned)T), dim3(256), 0, c->stream, scnt, n, spart, soff);
+        int32_t *d_err = nullptr, h_err = 0;
+        if (xb_scan_i32(c->stream, scnt, n, soff, reinterpret_cast<unsigned long long *>(spart),
+                        &d_err) != hipSuccess ||
+            hipMemcpyAsync(&h_err, d_err, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess || h_err)
+            rc = h_err ? XBFS_ERR_DEVICE : XBFS_ERR_HIP;
         long long last_off = 0;
         int32_t last_cnt = 0;
-        if (hipGetLastError() != hipSuccess ||
+        if (rc != XBFS_OK || hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(&last_off, soff + (n - 1), sizeof(long long), hipMemcpyDeviceToHost,
                            c->stream) != hipSuccess ||
             hipMemcpyAsync(&last_cnt, scnt + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost,
@@ -1081,25 +1108,26 @@ int xbfs_exclusive_scan_i32(xbfs_ctx *c, const int32_t *in, int64_t *out, int64_
     XB_HIP(hipSetDevice(c->device));
     const int64_t T = ceil_div(n, XB_SCAN_CHUNK);
     int32_t *d_in = nullptr;
-    long long *d_out = nullptr, *d_part = nullptr;
+    long long *d_out = nullptr;
+    unsigned long long *d_part = nullptr;
     int rc = XBFS_OK;
     if (hipMalloc(&d_in, sizeof(int32_t) * (size_t)n) != hipSuccess ||
         hipMalloc(&d_out, sizeof(long long) * (size_t)n) != hipSuccess ||
-        hipMalloc(&d_part, sizeof(long long) * (size_t)T) != hipSuccess) {
+        hipMalloc(&d_part, sizeof(unsigned long long) * (size_t)(T + 1)) != hipSuccess) {
         rc = XBFS_ERR_NOMEM;
     } else if (hipMemcpy(d_in, in, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice) !=
                hipSuccess) {
         rc = XBFS_ERR_HIP;
     } else {
-        hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)T), dim3(256), 0, c->stream, d_in, n,
-                           d_part);
-        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(1024), 0, c->stream, d_part, T);
-        hipLaunchKernelGGL(k_scan_down, dim3((unsigned)T), dim3(256), 0, c->stream, d_in, n,
-                           d_part, d_out);
-        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess ||
+        int32_t *d_err = nullptr, h_err = 0;
+        if (xb_scan_i32(c->stream, d_in, n, d_out, d_part, &d_err) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess ||
+            hipMemcpy(&h_err, d_err, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(out, d_out, sizeof(long long) * (size_t)n, hipMemcpyDeviceToHost) !=
                 hipSuccess)
             rc = XBFS_ERR_HIP;
+        else if (h_err)
+            rc = XBFS_ERR_DEVICE;
     }
     if (d_in) (void)hipFree(d_in);
     if (d_out) (void)hipFree(d_out);

@@ -6,8 +6,10 @@
 //                   ballot_counting + FQ_gen pair (bfs_adaptiveFQ_wb_async.cuh:
 //                   189-284) with my_prefixsum (prefix_sum1.cuh:59-108) moved
 //                   entirely onto the device.
-//   k_scan_reduce / k_scan_parts / k_scan_down
-//                   int32 -> int64 exclusive scan (prefix_sum1.cuh:13-108).
+//   k_scan_lookback
+//                   int32 -> int64 exclusive scan in one pass with decoupled
+//                   look-back (prefix_sum1.cuh:13-108, whose host-mediated
+//                   block sums it replaces).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "xbfs_device.h"
@@ -226,57 +228,91 @@ __global__ void __launch_bounds__(256) k_bu_scatter(const uint64_t *__restrict__
     }
 }
 
-// ---------------------------------------------------------------------------
-// generic int32 -> int64 exclusive scan (three passes)
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_scan_reduce(const int32_t *__restrict__ in, int64_t n,
-                                                     long long *__restrict__ part)
+// Single-pass exclusive scan of int32 into int64 with decoupled look-back
+// (Merrill and Garland, "Single-pass Parallel Prefix Scan with Decoupled
+// Look-back", NVIDIA technical report NVR-2016-002): one launch, every
+// element read once and written once.
+//  - Tiles of XB_SCAN_CHUNK elements are handed out by an atomic ticket in the
+//    order blocks start, so every tile's predecessors have already started
+//    and none of them waits on a later one: no dependence on which blocks
+//    the hardware runs first or at once.
+//  - Each tile sums its elements (thread t owns XB_SCAN_CHUNK/256 consecutive
+//    ones, block scan of the per-thread sums), publishes its aggregate
+//    (flag A), walks back over the preceding tiles' status words adding
+//    aggregates until it meets an inclusive prefix (flag P), and publishes
+//    its own inclusive prefix.
+//  - A status word packs the flag (bits 62-63) and the signed value (62-bit
+//    two's complement) in one 64-bit word, so one atomic load reads both.
+//  - The wait for a predecessor's word is bounded (2^26 polls); past it the
+//    tile sets *err and stops waiting.
+// status[T] and *ticket must be zero before the launch.
+#define XB_SCAN_FLAG_A 1ull
+#define XB_SCAN_FLAG_P 2ull
+__device__ __forceinline__ unsigned long long xb_scan_pack(unsigned long long flag, long long v)
 {
-    const int64_t lo = (int64_t)blockIdx.x * XB_SCAN_CHUNK;
-    long long s = 0;
-    for (int64_t i = lo + threadIdx.x; i < n && i < lo + XB_SCAN_CHUNK; i += blockDim.x) s += in[i];
-    long long tot = 0;
-    (void)xb_block_excl_scan_i64(s, &tot);
-    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+    return (flag << 62) | ((unsigned long long)v & ((1ull << 62) - 1));
+}
+__device__ __forceinline__ long long xb_scan_value(unsigned long long w)
+{
+    return (long long)(w << 2) >> 2;
 }
 
-__global__ void __launch_bounds__(1024) k_scan_parts(long long *__restrict__ part, int64_t T)
+__global__ void __launch_bounds__(256) k_scan_lookback(const int32_t *__restrict__ in, int64_t n,
+                                                       unsigned long long *status, int32_t *ticket,
+                                                       int32_t *err, long long *__restrict__ out)
 {
-    const int64_t per = (T + blockDim.x - 1) / blockDim.x;
-    const int64_t lo = (int64_t)threadIdx.x * per;
-    const int64_t hi = lo + per < T ? lo + per : T;
-    long long s = 0;
-    for (int64_t i = lo; i < hi; i++) s += part[i];
-    long long tot = 0;
-    long long run = xb_block_excl_scan_i64(s, &tot);
-    for (int64_t i = lo; i < hi; i++) {
-        long long t = part[i];
-        part[i] = run;
-        run += t;
-    }
-}
-
-// Each block scans its chunk: thread t owns XB_SCAN_CHUNK/256 consecutive
-// elements (LDS-free: per-thread serial sums, block scan of the sums).
-__global__ void __launch_bounds__(256) k_scan_down(const int32_t *__restrict__ in, int64_t n,
-                                                   const long long *__restrict__ part,
-                                                   long long *__restrict__ out)
-{
+    __shared__ int32_t s_tile;
+    __shared__ long long s_excl;
+    if (threadIdx.x == 0)
+        s_tile = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int64_t tile = s_tile;
     const int per = XB_SCAN_CHUNK / 256;
-    const int64_t lo = (int64_t)blockIdx.x * XB_SCAN_CHUNK + (int64_t)threadIdx.x * per;
+    const int64_t lo = tile * XB_SCAN_CHUNK + (int64_t)threadIdx.x * per;
+    int32_t x[XB_SCAN_CHUNK / 256];
     long long s = 0;
+#pragma unroll
     for (int k = 0; k < per; k++) {
-        int64_t i = lo + k;
-        if (i < n) s += in[i];
+        const int64_t i = lo + k;
+        x[k] = i < n ? in[i] : 0;
+        s += x[k];
     }
     long long tot = 0;
-    long long run = xb_block_excl_scan_i64(s, &tot) + part[blockIdx.x];
-    for (int k = 0; k < per; k++) {
-        int64_t i = lo + k;
-        if (i < n) {
-            out[i] = run;
-            run += in[i];
+    const long long ex = xb_block_excl_scan_i64(s, &tot);
+    if (threadIdx.x == 0) {
+        long long run = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&status[0], xb_scan_pack(XB_SCAN_FLAG_P, tot), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&status[tile], xb_scan_pack(XB_SCAN_FLAG_A, tot), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            for (int64_t j = tile - 1; j >= 0; j--) {
+                unsigned long long w = 0;
+                for (int poll = 0; poll < (1 << 26); poll++) {
+                    w = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (w >> 62) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                if ((w >> 62) == 0) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                run += xb_scan_value(w);
+                if ((w >> 62) == XB_SCAN_FLAG_P) break;
+            }
+            __hip_atomic_store(&status[tile], xb_scan_pack(XB_SCAN_FLAG_P, run + tot),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        s_excl = run;
+    }
+    __syncthreads();
+    long long r = s_excl + ex;
+#pragma unroll
+    for (int k = 0; k < per; k++) {
+        const int64_t i = lo + k;
+        if (i < n) out[i] = r;
+        r += x[k];
     }
 }
 
