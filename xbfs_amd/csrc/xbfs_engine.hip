@@ -990,13 +990,15 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
         }
         xb_wave_sync();
         // the wave owns the chunk's words (one cache line per bitmap) for
-        // the whole level: plain read-modify-write stores
+        // the whole level; atomic ORs whose results are not used cost no
+        // round trip, where a load-modify-store would stall the wave on the
+        // load before its next chunk
         if (ln < XB_CHUNK_WORDS && w < a.nwords) {
             const unsigned long long va = wl->va[ln];
             if (va) {
-                a.vis[w] |= va;
+                atomicOr((unsigned long long *)&a.vis[w], va);
                 const unsigned long long f1 = wl->f1[ln];
-                if (f1) F1[w] |= f1;
+                if (f1) atomicOr((unsigned long long *)&F1[w], f1);
             }
         }
     }
