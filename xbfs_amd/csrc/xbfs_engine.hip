@@ -612,11 +612,26 @@ __device__ __forceinline__ void td_visit(const XbArgs &a, const XbTdCtx &t, int3
         }
     }
     if (SF && t.enqueue) {
+        // wave-aggregated appends to the 4 degree-bin queues: the first lane of
+        // each bin (an active lane: callers run this inside loops that some
+        // lanes have left) reserves the wave's slots of that bin, the up to 4
+        // atomics in one instruction (one round trip rather than one per
+        // non-empty bin)
         const int b = won && d > 0 ? xb_bin_of(a, d) : -1;
-        xb_wave_append(t.qn[0], &t.cn[0].v, b == 0, v);
-        xb_wave_append(t.qn[1], &t.cn[1].v, b == 1, v);
-        xb_wave_append(t.qn[2], &t.cn[2].v, b == 2, v);
-        xb_wave_append(t.qn[3], &t.cn[3].v, b == 3, v);
+        const unsigned long long m0 = __ballot(b == 0), m1 = __ballot(b == 1);
+        const unsigned long long m2 = __ballot(b == 2), m3 = __ballot(b == 3);
+        if ((m0 | m1 | m2 | m3) != 0ull) {
+            const int ln = xb_lane();
+            const unsigned long long mb = b == 0 ? m0 : b == 1 ? m1 : b == 2 ? m2 : b == 3 ? m3 : 0ull;
+            const int lead = mb != 0ull ? __ffsll(mb) - 1 : 0;
+            int r = 0;
+            if (b >= 0 && ln == lead) r = atomicAdd(&t.cn[b].v, (int)__popcll(mb));
+            const int base = __shfl(r, lead);
+            if (b >= 0) {
+                // the bins' queues are consecutive n-entry slices (xb_q)
+                t.qn[0][(int64_t)b * a.n + base + xb_rank(mb)] = v;
+            }
+        }
     }
 }
 
