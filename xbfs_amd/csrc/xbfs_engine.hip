@@ -591,7 +591,12 @@ __device__ __forceinline__ void td_visit(const XbArgs &a, const XbTdCtx &t, int3
     bool won = false;
     int64_t d = 0;
     if (valid && a.level[v] == -1) {
+        int64_t rb = 0, re = 0;
         if (SF) {
+            // the row bounds are requested with the claim, so the two round
+            // trips overlap (only the winner uses them)
+            rb = a.beg[v];
+            re = a.beg[v + 1];
             won = atomicCAS(&a.level[v], -1, t.L1) == -1;
         } else {
             __builtin_nontemporal_store(t.L1, &a.level[v]);
@@ -605,7 +610,7 @@ __device__ __forceinline__ void td_visit(const XbArgs &a, const XbTdCtx &t, int3
                 const uint64_t bit = 1ull << (v & 63);
                 atomicOr((unsigned long long *)&a.vis[v >> 6], (unsigned long long)bit);
                 atomicOr((unsigned long long *)&t.Fn[v >> 6], (unsigned long long)bit);
-                d = a.beg[v + 1] - a.beg[v];
+                d = re - rb;
                 my_n += 1;
                 my_m += (unsigned long long)d;
             }
