@@ -379,6 +379,7 @@ static XbArgs make_args(xbfs_ctx *c, int32_t src, const xbfs_opts &o)
     a.srcs = nullptr;
     a.nsrc = 0;
     a.hub_first = c->hub_first;
+    a.nt_bu = 8 * (long long)c->n > XB_NT_HUB_BYTES ? 1 : 0;
     a.src = src;
     a.t1 = o.bin_thread_max;
     a.t2 = o.bin_wave_max;

@@ -283,7 +283,10 @@ __device__ void ph_init(const XbArgs &a, int bid, int nb)
             else if (k == 2) { v.z = 0; p.z = s; }
             else { v.w = 0; p.w = s; }
         }
-        l4[i] = v;
+        // non-temporal: 4 bytes per vertex written once, not read back here
+        typedef int xb_v4i __attribute__((ext_vector_type(4)));
+        const xb_v4i vv = {v.x, v.y, v.z, v.w};
+        __builtin_nontemporal_store(vv, reinterpret_cast<xb_v4i *>(&l4[i]));
         if (a.parent) p4[i] = p;
     }
     for (int64_t i = n4 * 4 + gtid; i < a.n; i += nthr) {
@@ -854,15 +857,14 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                 const int i = r + 64 * t + ln;
                 vc[t] = i < total ? wl->list[i] : -1;
             }
-            // hub records and row starts are read once per level: non-temporal
-            // loads keep them from evicting the frontier bitmap the probes hit
-            // in L2 (RMAT-24: 2 % less time per traversal; the adjacency
-            // words, whose lines short rows share, stay cached)
+            // hub records (and row starts below) are read once per level:
+            // non-temporal loads on large graphs (a.nt_bu, XB_NT_HUB_BYTES);
+            // the adjacency words, whose lines short rows share, stay cached
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const long long hv =
-                    vc[t] >= 0 ? __builtin_nontemporal_load(reinterpret_cast<const long long *>(&a.hd[vc[t]]))
-                               : (long long)0xffffffffll; // {-1, 0}
+                const long long *hp = reinterpret_cast<const long long *>(&a.hd[vc[t] >= 0 ? vc[t] : 0]);
+                const long long hv = vc[t] < 0 ? (long long)0xffffffffll // {-1, 0}
+                                     : a.nt_bu ? __builtin_nontemporal_load(hp) : *hp;
                 h[t] = make_int2((int32_t)(hv & 0xffffffffll), (int32_t)(hv >> 32));
             }
             // 32-bit probes: the dword of the bitmap word that holds the bit
@@ -920,8 +922,8 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
                 const int k = next + xb_rank(im);
                 if (k < nres) {
                     v = wl->list[k];
-                    b = __builtin_nontemporal_load(&a.beg[v]);
-                    e = __builtin_nontemporal_load(&a.beg[v + 1]);
+                    b = a.nt_bu ? __builtin_nontemporal_load(&a.beg[v]) : a.beg[v];
+                    e = a.nt_bu ? __builtin_nontemporal_load(&a.beg[v + 1]) : a.beg[v + 1];
                     j = b + skip;
                     ce = (e - j > XB_BU_LANE_MAX) ? j + XB_BU_LANE_MAX : e;
                 }

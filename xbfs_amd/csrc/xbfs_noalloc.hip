@@ -183,6 +183,7 @@ extern "C" int xbfs_bfs_noalloc(int device, const int64_t *d_beg, const int32_t 
     a.srcs = nullptr;
     a.nsrc = 0;
     a.hub_first = 0;
+    a.nt_bu = 8 * (long long)n > XB_NT_HUB_BYTES ? 1 : 0;
     a.src = src;
     a.t1 = o.bin_thread_max;
     a.t2 = o.bin_wave_max;
