@@ -74,8 +74,8 @@ HBM_PEAK = 8.0e12
 # moves: factor 2.0; WRITE_SIZE reads all of them: factor 1.0).  kernel_us is
 # the mean launch time of the kernel trace taken with them; the figure is
 # reported only while the measured launch time stays within 10 % of it.
-PMC_TRAFFIC = {"workload": (24, 16, 64, "learned"), "fetch_kib": 410486.8, "fetch_factor": 1.99995,
-               "write_kib": 257608.5, "write_factor": 1.0, "kernel_us": 424.9,
+PMC_TRAFFIC = {"workload": (24, 16, 64, "learned"), "fetch_kib": 405026.9, "fetch_factor": 1.99994,
+               "write_kib": 258182.3, "write_factor": 1.0, "kernel_us": 405.5,
                "profile": "profiles/k_persistent_rmat24_pmc_traffic.md"}
 
 MODE_SF, MODE_SS, MODE_BU = 0, 1, 2
