@@ -173,6 +173,23 @@ def test_launch_kinds(monkeypatch):
         ctx.close()
 
 
+def test_nontemporal_bottom_up_loads(monkeypatch):
+    """Bottom-up phases read the hub records and row starts with
+    non-temporal loads on graphs whose hub table exceeds 64 MB and with plain
+    loads below; XBFS_NT_BU=1 / 0 forces either on a context.  A graph of
+    either size gives the oracle's levels and parents both ways."""
+    g = Graph.rmat(16, 16, seed=1016)
+    srcs = seeded_sources(g, 3)
+    for env in ("1", "0"):
+        monkeypatch.setenv("XBFS_NT_BU", env)
+        ctx = Context(g)
+        for s in srcs:
+            for pol in (dict(policy=POLICY_HEURISTIC), dict(policy=POLICY_FORCE_BU)):
+                say("nt_bu", env, s, pol)
+                _check_one(ctx, g, s, persistent=1, **pol)
+        ctx.close()
+
+
 def test_hub_component_termination(monkeypatch):
     """A traversal from the largest-degree vertex's component ends as soon as
     every member has its level, one level (or, after +2 discoveries, two)

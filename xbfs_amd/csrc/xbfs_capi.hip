@@ -229,6 +229,7 @@ int xbfs_create(int device, const int64_t *beg, const int32_t *adj, int64_t n, i
     if (clk_khz > 0) c->wall_hz = (double)clk_khz * 1e3;
     if (const char *e = getenv("XBFS_LAUNCH")) c->launch_plain = strcmp(e, "coop") != 0;
     if (const char *e = getenv("XBFS_RESIDENCY_US")) c->res_us = atof(e);
+    if (const char *e = getenv("XBFS_NT_BU")) c->nt_bu = atoi(e) != 0 ? 1 : 0;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent, XB_PERSIST_BLOCK,
                                                      XB_LDS_PERSIST) !=
@@ -379,7 +380,7 @@ static XbArgs make_args(xbfs_ctx *c, int32_t src, const xbfs_opts &o)
     a.srcs = nullptr;
     a.nsrc = 0;
     a.hub_first = c->hub_first;
-    a.nt_bu = 8 * (long long)c->n > XB_NT_HUB_BYTES ? 1 : 0;
+    a.nt_bu = c->nt_bu >= 0 ? c->nt_bu : (8 * (long long)c->n > XB_NT_HUB_BYTES ? 1 : 0);
     a.src = src;
     a.t1 = o.bin_thread_max;
     a.t2 = o.bin_wave_max;

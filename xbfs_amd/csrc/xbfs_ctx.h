@@ -39,6 +39,7 @@ struct xbfs_ctx {
     int32_t hub_first = 0;      // d_adj_bu is built (each row starts with its hub)
     int launch_plain = 1;       // persistent kernel: plain launch with a residency check
     double res_us = 20000.0;    // bound of that check (first barrier wait), microseconds
+    int nt_bu = -1;             // bottom-up non-temporal loads: -1 by size (XB_NT_HUB_BYTES), 0 off, 1 on
     int64_t plain_fallbacks = 0; // plain launches that were relaunched cooperatively
     uint64_t *d_vis = nullptr;  // visited bitmap
     uint64_t *d_fbm = nullptr;  // 4 frontier bitmaps (ring by level & 3)
