@@ -19,11 +19,10 @@
 #define XB_PERSIST_BLOCK 1024   /* block size of the persistent kernel (1 per CU) */
 // Bottom-up phases read the hub records and, in scans, the row starts once
 // per level.  When the hub table (8 bytes per vertex) exceeds this size, a
-// quarter of MI355X's 256 MB Infinity Cache, those loads are non-temporal, so
-// the stream does not evict the frontier bitmap the probes hit in L2; below it
-// the table stays cached from one level to the next and is read normally.
-// (RMAT-24, 134 MB: 2 % less time per traversal; RMAT-22, 34 MB: 3.5 % more
-// with non-temporal loads; the 256-graph multigraph, 35 MB: 9 % more.)
+// quarter of MI355X's 256 MB Infinity Cache, those loads are non-temporal (a
+// stream that is not re-read within the level); below it they are plain
+// loads.  Switched at run time within one build (XBFS_NT_BU), the two kinds
+// time the same on RMAT-22, RMAT-23, RMAT-24 and Erdos-Renyi 2^24 graphs.
 #define XB_NT_HUB_BYTES (64ll << 20)
 #ifndef XB_BU_LANE_MAX
 #define XB_BU_LANE_MAX 32       /* bottom-up: edges one lane scans before the wave takes over */

@@ -859,7 +859,7 @@ __device__ void ph_bu(const XbArgs &a, int32_t L, int bid, int nb, XbWaveLds *ld
             }
             // hub records (and row starts below) are read once per level:
             // non-temporal loads on large graphs (a.nt_bu, XB_NT_HUB_BYTES);
-            // the adjacency words, whose lines short rows share, stay cached
+            // the adjacency words, whose lines short rows share, are plain
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 const long long *hp = reinterpret_cast<const long long *>(&a.hd[vc[t] >= 0 ? vc[t] : 0]);
