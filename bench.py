@@ -259,6 +259,9 @@ def main():
                          "(profiling runs, so every k_persistent launch is the measured one)")
     ap.add_argument("--batch", type=int, default=1,
                     help="1 = also measure the 64-source batch and the 256-graph multigraph batch")
+    ap.add_argument("--batch-only", type=int, default=0, metavar="N",
+                    help="run only N launches of the 64-source batch on RMAT-22 ef16 (after one "
+                         "warmup launch), print their times, and exit (profiling k_b64_persistent)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="run only a plain device-to-device copy of BYTES bytes, 5 times "
                          "(scale factor for the FETCH_SIZE / WRITE_SIZE counters), and exit")
@@ -276,6 +279,23 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": a.numel() * 4, "launches": 5}))
+        return
+
+    if args.batch_only > 0:
+        from xbfs_amd import Context, Graph
+        g = Graph.rmat(22, 16, 0.57, 0.19, 0.19, seed=1022)
+        ctx = Context(g, device=0)
+        srcs = pick_sources(g.degrees(), 64)
+        ctx.bfs_batch64(srcs, want_levels=False)
+        ts, st = [], None
+        for _ in range(args.batch_only):
+            _, st = ctx.bfs_batch64(srcs, want_levels=False)
+            ts.append(st["t_bfs_s"])
+        ctx.close()
+        print(json.dumps({"batch64_only": {
+            "workload": "RMAT scale 22 ef16 seed 1022, 64 seeded sources", "n": g.n, "m": g.m,
+            "launches": len(ts) + 1, "batch_ms_median": round(float(np.median(ts)) * 1e3, 3),
+            "levels": st["n_levels"], "level_rows_bytes": 64 * 4 * g.n}}))
         return
 
     from xbfs_amd import Context, Graph, default_opts, POLICY_HEURISTIC, POLICY_LEARNED
